@@ -1,0 +1,209 @@
+/*
+ * mpskit_hip.h -- C ABI of the MI355X (gfx950) effective-Hamiltonian and
+ * transfer-matrix contraction library.
+ *
+ * Drop-in boundary for MPSKit.jl v0.13.13's derivative operators, transfer
+ * matrices and environment updates.  Every entry point below names the
+ * reference routine it replaces (paths relative to the MPSKit.jl tree).
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer to float64 data in the memory
+ *     layout of a Trivial-symmetry TensorKit.TensorMap: one column-major array
+ *     with axes (codomain..., domain...):
+ *        GL[a,g,f]   (D_bra x w <- D_ket)        src/environments/abstract_envs.jl:36-37
+ *        GR[i,l,c]   (D_ket x w <- D_bra)        src/environments/abstract_envs.jl:50-51
+ *        W[g,b,h,k]  (w_l x p_out <- p_in x w_r)  src/states/abstractmps.jl:8-11
+ *        A[l,p,r]    (D_l x p <- D_r)
+ *        AC2 x[f,h,i,j] (D_l x p1 <- D_r x p2)    src/states/finitemps.jl:376-384
+ *     so a Julia caller passes `pointer(t.data)` of a device-adapted tensor.
+ *   - MPO tensors are dense (materialised JordanMPOTensor / SparseBlockTensorMap).
+ *     For the Jordan ("MPOHamiltonian") entry points, channel 0 of an MPO bond
+ *     is "not started", the last channel "finished", the channels in between
+ *     the A-block (src/operators/jordanmpotensor.jl:1-69); a left boundary
+ *     bond of dimension 1 holds only "not started", a right boundary bond of
+ *     dimension 1 only "finished".
+ *   - Work is enqueued on the context's stream (mk_ctx_set_stream); calls
+ *     return once enqueued unless documented otherwise.  Prepared operator
+ *     handles are immutable; applies of operators that share a context must
+ *     be issued from one host thread at a time (the context owns the scratch
+ *     arena, which is stream-ordered).  Use one context per concurrent caller.
+ *   - Status codes: MK_OK on success; otherwise mk_last_error() returns a
+ *     thread-local message.  Mirrors the reference's SpaceMismatch /
+ *     ArgumentError throws (src/utility/utility.jl:276-277, 359-360).
+ */
+#ifndef MPSKIT_HIP_H
+#define MPSKIT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mk_status {
+  MK_OK = 0,
+  MK_BAD_SHAPE = 1,      /* SpaceMismatch analogue */
+  MK_BAD_ARG = 2,        /* ArgumentError analogue */
+  MK_OOM = 3,
+  MK_HIP_ERROR = 4,
+  MK_NOT_CONVERGED = 5,  /* KrylovKit non-convergence warning analogue */
+  MK_NOT_SUPPORTED = 6,
+  MK_NONFINITE = 7       /* NaN/Inf met in an input or iterate */
+} mk_status;
+
+/* Arithmetic used for the bulk contractions of an apply.  FP64 is the
+ * reference arithmetic; FP32 runs the GEMMs on FP32 MFMA; BF16X3 splits each
+ * float64 operand into bf16 hi+lo and accumulates hi*hi + hi*lo + lo*hi in
+ * FP32 on BF16 MFMA.  Inputs and outputs stay float64 in every mode. */
+typedef enum mk_precision { MK_FP64 = 0, MK_FP32 = 1, MK_BF16X3 = 2 } mk_precision;
+/* bit masks for mk_prepare_* "precisions" arguments */
+#define MK_PREC_FP64 (1 << MK_FP64)
+#define MK_PREC_FP32 (1 << MK_FP32)
+#define MK_PREC_BF16X3 (1 << MK_BF16X3)
+
+typedef struct mk_ctx_s* mk_ctx;
+typedef struct mk_op_s* mk_op;
+
+/* ---- library / context -------------------------------------------------- */
+int mk_version(void);
+const char* mk_last_error(void);
+mk_status mk_ctx_create(int device, mk_ctx* out);
+mk_status mk_ctx_destroy(mk_ctx ctx);
+/* stream is a hipStream_t (NULL = legacy default stream) */
+mk_status mk_ctx_set_stream(mk_ctx ctx, void* stream);
+mk_status mk_ctx_synchronize(mk_ctx ctx);
+
+/* ---- derivative operators (prepare) -------------------------------------- */
+
+/* Zero-site derivative, replaces MPO_C_Hamiltonian + prepare_operator!!
+ * (src/algorithms/derivatives/mpo_derivatives.jl:37-44, :92-97, :216-220,
+ *  PrecomputedCDerivative :249-264).
+ * GL (Da x w x Df), GR (Di x w x Dc);  x (Df x Di) -> y (Da x Dc). */
+mk_status mk_prepare_c(mk_ctx ctx, const double* GL, const double* GR, int64_t Da,
+                       int64_t Df, int64_t w, int64_t Di, int64_t Dc, int precisions,
+                       mk_op* out);
+
+/* Single-site generic derivative, replaces MPO_AC_Hamiltonian + prepare
+ * (mpo_derivatives.jl:46-54, :99-106, :221-231, PrecomputedACDerivative :266-274).
+ * GL (Da x wl x Df), W (wl x d x d x wr), GR (Di x wr x Dc);
+ * x (Df x d x Di) -> y (Da x d x Dc). */
+mk_status mk_prepare_ac_generic(mk_ctx ctx, const double* GL, const double* W,
+                                const double* GR, int64_t Da, int64_t Df, int64_t wl,
+                                int64_t d, int64_t wr, int64_t Di, int64_t Dc,
+                                int precisions, mk_op* out);
+
+/* Two-site generic derivative, replaces MPO_AC2_Hamiltonian + prepare
+ * (mpo_derivatives.jl:56-66, :144-152, :233-247, PrecomputedAC2Derivative :276-288).
+ * GL (Da x wl x Df), W1 (wl x d1 x d1 x wm), W2 (wm x d2 x d2 x wr), GR (Di x wr x Dc);
+ * x (Df x d1 x Di x d2) -> y (Da x d1 x Dc x d2). */
+mk_status mk_prepare_ac2_generic(mk_ctx ctx, const double* GL, const double* W1,
+                                 const double* W2, const double* GR, int64_t Da,
+                                 int64_t Df, int64_t wl, int64_t d1, int64_t wm,
+                                 int64_t d2, int64_t wr, int64_t Di, int64_t Dc,
+                                 int precisions, mk_op* out);
+
+/* Single-site Jordan-block (MPOHamiltonian) derivative, replaces
+ * JordanMPO_AC_Hamiltonian ctor + prepare_operator!!
+ * (src/algorithms/derivatives/hamiltonian_derivatives.jl:44-55, :57-107, :109-157,
+ *  apply :472-483).  Bra == ket: GL (Dl x wl x Dl), W (wl x d x d x wr),
+ * GR (Dr x wr x Dr); x (Dl x d x Dr) -> y (Dl x d x Dr).  As in the reference
+ * the "not started" left level and the "finished" right level of the
+ * environments are taken to be identities. */
+mk_status mk_prepare_ac_jordan(mk_ctx ctx, const double* GL, const double* W,
+                               const double* GR, int64_t Dl, int64_t wl, int64_t d,
+                               int64_t wr, int64_t Dr, int precisions, mk_op* out);
+
+/* Two-site Jordan-block derivative, replaces JordanMPO_AC2_Hamiltonian ctor +
+ * prepare_operator!! (hamiltonian_derivatives.jl:206-217, :243-270, :272-369,
+ * :371-468; apply :485-500).  GL (Dl x wl x Dl), W1 (wl x d1 x d1 x wm),
+ * W2 (wm x d2 x d2 x wr), GR (Dr x wr x Dr); x (Dl x d1 x Dr x d2) -> y. */
+mk_status mk_prepare_ac2_jordan(mk_ctx ctx, const double* GL, const double* W1,
+                                const double* W2, const double* GR, int64_t Dl,
+                                int64_t wl, int64_t d1, int64_t wm, int64_t d2,
+                                int64_t wr, int64_t Dr, int precisions, mk_op* out);
+
+/* ---- derivative operators (apply) ---------------------------------------- */
+
+/* y = alpha * H(x) + beta * y using arithmetic `precision` (which must have
+ * been requested at prepare).  Replaces `(H::DerivativeOperator)(x)` / `H * x`
+ * (src/algorithms/derivatives/derivatives.jl:8-11).  x and y must not alias. */
+mk_status mk_apply(mk_op op, const double* x, double* y, double alpha, double beta,
+                   int precision);
+
+/* number of float64 elements of x and y, and the algorithmic flop count of one
+ * apply (each contraction counted once, whatever the precision scheme) */
+mk_status mk_op_info(mk_op op, int64_t* x_elems, int64_t* y_elems, double* flops);
+mk_status mk_op_free(mk_op op);
+
+/* ---- transfer matrices / environment updates ---------------------------- */
+
+/* transfer_left(v, A, Abar), bond-tensor v (src/transfermatrix/transfer.jl:18-31):
+ * y[a,c] = v[m,n] A[n,h,c] Abar[m,h,a];  v (Dm x Dn), A (Dn x d x Dc),
+ * Abar (Dm x d x Da) -> y (Da x Dc). */
+mk_status mk_transfer_left_mps(mk_ctx ctx, const double* v, const double* A,
+                               const double* Abar, int64_t Dm, int64_t Dn, int64_t d,
+                               int64_t Dc, int64_t Da, double* y);
+
+/* transfer_right(v, A, Abar) (transfer.jl:43-55):
+ * y[a,b] = A[a,h,i] Abar[b,h,j] v[i,j];  A (Da x d x Di), Abar (Db x d x Dj),
+ * v (Di x Dj) -> y (Da x Db). */
+mk_status mk_transfer_right_mps(mk_ctx ctx, const double* v, const double* A,
+                                const double* Abar, int64_t Da, int64_t d, int64_t Di,
+                                int64_t Db, int64_t Dj, double* y);
+
+/* MPO transfer_left(x, O, A, Abar) (transfer.jl:125-127); the finite left
+ * environment update GLs[j+1] = GLs[j] * TransferMatrix(AL, W, AL)
+ * (src/environments/finite_envs.jl:137-150, transfermatrix.jl:48-54):
+ * y[a,b,c] = x[m,g,n] A[n,h,c] O[g,k,h,b] Abar[m,k,a];
+ * x (Dm x wl x Dn), O (wl x d x d x wr), A (Dn x d x Dc), Abar (Dm x d x Da)
+ * -> y (Da x wr x Dc). */
+mk_status mk_transfer_left_mpo(mk_ctx ctx, const double* x, const double* O,
+                               const double* A, const double* Abar, int64_t Dm,
+                               int64_t wl, int64_t Dn, int64_t d, int64_t wr,
+                               int64_t Dc, int64_t Da, double* y);
+
+/* MPO transfer_right(v, O, A, Abar) (transfer.jl:128-130); the finite right
+ * environment update (finite_envs.jl:110-128):
+ * y[a,b,c] = A[a,h,i] O[b,k,h,l] Abar[c,k,o] v[i,l,o];
+ * A (Da x d x Di), O (wl x d x d x wr), Abar (Dc x d x Do), v (Di x wr x Do)
+ * -> y (Da x wl x Dc). */
+mk_status mk_transfer_right_mpo(mk_ctx ctx, const double* v, const double* O,
+                                const double* A, const double* Abar, int64_t Da,
+                                int64_t d, int64_t Di, int64_t wl, int64_t wr,
+                                int64_t Dc, int64_t Do, double* y);
+
+/* regularize!(v, lvec, rvec) (src/transfermatrix/transfermatrix.jl:70-80), in
+ * place, for v (D1 x w x D2) (w = 1: a bond tensor):
+ * v[a,g,c] -= (sum_ij lvec[i,j] v[j,g,i]) rvec[a,c];  lvec (D2 x D1), rvec (D1 x D2). */
+mk_status mk_regularize(mk_ctx ctx, double* v, const double* lvec, const double* rvec,
+                        int64_t D1, int64_t w, int64_t D2);
+
+/* ---- Krylov eigensolver ---------------------------------------------------- */
+
+/* Lowest (:SR) eigenpair of a prepared Hermitian derivative operator by
+ * Lanczos with full FP64 reorthogonalisation on the device; replaces
+ * fixedpoint(H, x0, :SR, Lanczos(...)) (src/algorithms/fixedpoint.jl:15-18,
+ * KrylovKit.eigsolve).  x: in = start vector, out = normalised eigenvector.
+ * Restarts after `krylovdim` vectors, at most `maxiter` restarts; converged
+ * when the Ritz residual norm <= tol.  `precision` selects the apply
+ * arithmetic; for MK_FP32/MK_BF16X3 every restart is closed by an FP64 apply
+ * and the reported eigenvalue and residual are FP64 (iterative refinement).
+ * Blocks the host thread (one scalar readback per Lanczos step). */
+mk_status mk_eigsolve(mk_op op, double* x, int krylovdim, int maxiter, double tol,
+                      int precision, double* eigval, double* resnorm, int* numops,
+                      int* converged);
+
+/* ---- kernel timing ---------------------------------------------------------
+ * No counterpart in the reference (its TimerOutputs sections time whole
+ * algorithm steps, src/algorithms/groundstate/dmrg.jl:139-150).  When enabled,
+ * every GEMM launch of the library is bracketed by HIP events recorded on its
+ * own stream; mk_prof_query returns the summed device time and launch count of
+ * one kernel family ("gemm_f64", "gemm_f32", "gemm_bf16x3"). */
+mk_status mk_prof_enable(int on);
+mk_status mk_prof_reset(void);
+mk_status mk_prof_query(const char* kernel, double* total_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPSKIT_HIP_H */

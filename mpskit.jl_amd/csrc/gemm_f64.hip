@@ -1,0 +1,302 @@
+// FP64 MFMA GEMM for CDNA4 (gfx950): v_mfma_f64_16x16x4_f64, LDS-staged
+// BK=16 tiles, double-buffered through registers, 256 threads = 4 waves of 64
+// lanes in a 2x2 arrangement.  One workgroup computes one BM x BN output tile
+// of one problem, summing every segment of that problem into the same
+// accumulators before a single fused epilogue (alpha/beta and optional CB term).
+#include "gemm.h"
+#include "prof.h"
+
+namespace mk {
+namespace {
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef double dx2 __attribute__((ext_vector_type(2)));
+
+constexpr int NT = 256;
+constexpr int BK = 16;
+constexpr int PAD = 16;     // LDS row padding (doubles): alternates 128-B bank halves
+constexpr int NXCD = 8;
+constexpr int GROUP_M = 8;  // tile-row grouping for L2 reuse
+
+// Workgroups are dispatched round-robin over the XCDs.  Give XCD x a
+// contiguous range of tile ids so that neighbouring tiles (which share A row
+// panels / B column panels) are resident in the same L2.
+__device__ __forceinline__ int xcd_remap(int t, int total) {
+  const int q = total / NXCD, r = total % NXCD;
+  const int x = t % NXCD, local = t / NXCD;
+  return x * q + min(x, r) + local;
+}
+
+// Stage a tile whose contiguous memory direction is the LDS fast direction.
+// element (r, k) lives at p[(r0 + r) + (k0 + k) * ld]; LDS tile S[k][r].
+template <int R, int LDR>
+__device__ __forceinline__ void load_fastdir(const double* __restrict__ p, int64_t ld, int r0,
+                                             int k0, int Rlim, int Klim, bool vec,
+                                             dx2 (&reg)[R * BK / 2 / NT]) {
+  constexpr int CH = R * BK / 2 / NT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * NT;
+    const int k = id / (R / 2);
+    const int r = (id % (R / 2)) * 2;
+    const int gr = r0 + r, gk = k0 + k;
+    dx2 v = {0.0, 0.0};
+    if (gk < Klim) {
+      const double* q = p + (int64_t)gr + (int64_t)gk * ld;
+      if (vec && gr + 1 < Rlim) {
+        v = *reinterpret_cast<const dx2*>(q);
+      } else {
+        if (gr < Rlim) v.x = q[0];
+        if (gr + 1 < Rlim) v.y = q[1];
+      }
+    }
+    reg[c] = v;
+  }
+}
+
+template <int R, int LDR>
+__device__ __forceinline__ void store_fastdir(double (*S)[LDR], const dx2 (&reg)[R * BK / 2 / NT]) {
+  constexpr int CH = R * BK / 2 / NT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * NT;
+    const int k = id / (R / 2);
+    const int r = (id % (R / 2)) * 2;
+    *reinterpret_cast<dx2*>(&S[k][r]) = reg[c];
+  }
+}
+
+// Stage a tile whose contiguous memory direction is k.
+// element (r, k) lives at p[(k0 + k) + (r0 + r) * ld]; LDS tile S[k][r].
+template <int R, int LDR>
+__device__ __forceinline__ void load_kdir(const double* __restrict__ p, int64_t ld, int r0, int k0,
+                                          int Rlim, int Klim, bool vec,
+                                          dx2 (&reg)[R * BK / 2 / NT]) {
+  constexpr int CH = R * BK / 2 / NT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * NT;
+    const int r = id / (BK / 2);
+    const int k = (id % (BK / 2)) * 2;
+    const int gr = r0 + r, gk = k0 + k;
+    dx2 v = {0.0, 0.0};
+    if (gr < Rlim) {
+      const double* q = p + (int64_t)gk + (int64_t)gr * ld;
+      if (vec && gk + 1 < Klim) {
+        v = *reinterpret_cast<const dx2*>(q);
+      } else {
+        if (gk < Klim) v.x = q[0];
+        if (gk + 1 < Klim) v.y = q[1];
+      }
+    }
+    reg[c] = v;
+  }
+}
+
+template <int R, int LDR>
+__device__ __forceinline__ void store_kdir(double (*S)[LDR], const dx2 (&reg)[R * BK / 2 / NT]) {
+  constexpr int CH = R * BK / 2 / NT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * NT;
+    const int r = id / (BK / 2);
+    const int k = (id % (BK / 2)) * 2;
+    S[k][r] = reg[c].x;
+    S[k + 1][r] = reg[c].y;
+  }
+}
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int CHA = BM * BK / 2 / NT;
+  constexpr int CHB = BN * BK / 2 / NT;
+  static_assert(CHA >= 1 && CHB >= 1, "tile too small for 256 threads");
+
+  __shared__ double As[2][BK][LDA_S];
+  __shared__ double Bs[2][BK][LDB_S];
+
+  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
+  int pi = 0;
+  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
+  const GemmProb& P = batch.p[pi];
+
+  // grouped tile order inside the problem
+  const int u = t - P.tile_start;
+  const int per_group = GROUP_M * P.tiles_n;
+  const int g = u / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsize = min(GROUP_M, P.tiles_m - first_m);
+  const int tm = first_m + (u % per_group) % gsize;
+  const int tn = (u % per_group) / gsize;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+  const int li = lane & 15;   // row (A) / column (B, D) inside a 16x16 block
+  const int lk = lane >> 4;   // k slot (A, B) / row quad (D)
+
+  dx4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
+
+  int nsteps = 0;
+  for (int s = 0; s < P.nseg; ++s) nsteps += (P.seg[s].K + BK - 1) / BK;
+
+  dx2 ra[CHA], rb[CHB];
+  auto load = [&](int s, int k0) {
+    const GemmSeg& S = P.seg[s];
+    const bool vec = S.vec != 0;
+    const double* A = static_cast<const double*>(S.A);
+    const double* B = static_cast<const double*>(S.B);
+    if (!TA) load_fastdir<BM, LDA_S>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
+    else     load_kdir<BM, LDA_S>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
+    if (!TB) load_kdir<BN, LDB_S>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
+    else     load_fastdir<BN, LDB_S>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
+  };
+  auto store = [&](int buf) {
+    if (!TA) store_fastdir<BM, LDA_S>(As[buf], ra);
+    else     store_kdir<BM, LDA_S>(As[buf], ra);
+    if (!TB) store_kdir<BN, LDB_S>(Bs[buf], rb);
+    else     store_fastdir<BN, LDB_S>(Bs[buf], rb);
+  };
+
+  int s = 0, k0 = 0;
+  if (nsteps > 0) {
+    load(0, 0);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    int ns = s, nk0 = k0 + BK;
+    if (nk0 >= P.seg[s].K) { ns = s + 1; nk0 = 0; }
+    const bool has_next = step + 1 < nsteps;
+    if (has_next) load(ns, nk0);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      double a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[i] = As[buf][kk * 4 + lk][wm0 + i * 16 + li];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[j] = Bs[buf][kk * 4 + lk][wn0 + j * 16 + li];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (has_next) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+    s = ns;
+    k0 = nk0;
+  }
+
+  // epilogue: D lane layout -- column li, rows 4*lk + r
+  const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn0 + j * 16 + li;
+      if (col >= P.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm0 + i * 16 + lk * 4 + r;
+        if (row >= P.M) continue;
+        double v = acc[i][j][r];
+        if (P.epi == EPI_CB) {
+          const int a = row % P.Dl, bb = row / P.Dl;
+          const int c = col % P.Dr, e = col / P.Dr;
+          double tsum = 0.0;
+          for (int jj = 0; jj < P.d2; ++jj)
+            for (int h = 0; h < P.d1; ++h)
+              tsum += P.cb[bb + P.d1 * (e + P.d2 * (h + P.d1 * jj))] *
+                      P.cbx[a + (int64_t)P.Dl * (h + P.d1 * ((int64_t)c + (int64_t)P.Dr * jj))];
+          v += tsum;
+        }
+        double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
+        v *= alpha;
+        if (beta != 0.0) v += beta * *cp;
+        *cp = v;
+      }
+    }
+  }
+}
+
+template <int BM, int BN>
+mk_status launch_cfg(const ProbSpec* probs, int nprob, GemmShape sh, hipStream_t stream) {
+  GemmBatch b{};
+  int total = 0;
+  for (int i = 0; i < nprob; ++i) {
+    const ProbSpec& ps = probs[i];
+    GemmProb& P = b.p[i];
+    P.M = (int32_t)ps.M;
+    P.N = (int32_t)ps.N;
+    P.nseg = 0;
+    for (int s = 0; s < ps.nseg; ++s) {
+      if (ps.seg[s].K <= 0) continue;
+      GemmSeg& g = P.seg[P.nseg++];
+      g.A = ps.seg[s].A;
+      g.B = ps.seg[s].B;
+      g.Alo = g.Blo = nullptr;
+      g.lda = ps.seg[s].lda;
+      g.ldb = ps.seg[s].ldb;
+      g.K = (int32_t)ps.seg[s].K;
+      const bool al = ((uintptr_t)g.A % 16 == 0) && ((uintptr_t)g.B % 16 == 0) &&
+                      (g.lda % 2 == 0) && (g.ldb % 2 == 0);
+      g.vec = al ? 1 : 0;
+    }
+    P.tiles_m = (int32_t)cdiv(ps.M, BM);
+    P.tiles_n = (int32_t)cdiv(ps.N, BN);
+    P.tile_start = total;
+    total += P.tiles_m * P.tiles_n;
+    P.C = ps.C;
+    P.ldc = ps.ldc;
+    P.alpha = ps.alpha;
+    P.beta = ps.beta;
+    P.epi = ps.epi;
+    P.Dl = ps.Dl; P.Dr = ps.Dr; P.d1 = ps.d1; P.d2 = ps.d2;
+    P.cb = ps.cb;
+    P.cbx = ps.cbx;
+  }
+  b.nprob = nprob;
+  b.total_tiles = total;
+  if (total == 0) return MK_OK;
+  dim3 grid(total), block(NT);
+  ProfScope prof("gemm_f64", stream);
+  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false>), grid, block, 0, stream, b);
+  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false>), grid, block, 0, stream, b);
+  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true>), grid, block, 0, stream, b);
+  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true>), grid, block, 0, stream, b);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+}  // namespace
+
+mk_status gemm_f64(const ProbSpec* probs, int nprob, GemmShape shape, hipStream_t stream) {
+  MK_REQUIRE(nprob >= 0 && nprob <= MAX_PROB, MK_BAD_ARG, "gemm_f64: %d problems (max %d)", nprob, MAX_PROB);
+  int64_t tiles128 = 0;
+  for (int i = 0; i < nprob; ++i) {
+    MK_REQUIRE(probs[i].nseg <= MAX_SEG, MK_BAD_ARG, "gemm_f64: too many segments");
+    MK_REQUIRE(probs[i].M >= 0 && probs[i].N >= 0 && probs[i].M < (1ll << 31) && probs[i].N < (1ll << 31),
+               MK_BAD_SHAPE, "gemm_f64: bad M/N");
+    for (int s = 0; s < probs[i].nseg; ++s)
+      MK_REQUIRE(probs[i].seg[s].K >= 0 && probs[i].seg[s].K < (1ll << 31), MK_BAD_SHAPE, "gemm_f64: bad K");
+    tiles128 += cdiv(probs[i].M, 128) * cdiv(probs[i].N, 128);
+  }
+  // 128x128 tiles once they fill the 256 CUs, else 64x64 for more parallelism
+  if (tiles128 >= 256) return launch_cfg<128, 128>(probs, nprob, shape, stream);
+  return launch_cfg<64, 64>(probs, nprob, shape, stream);
+}
+
+}  // namespace mk

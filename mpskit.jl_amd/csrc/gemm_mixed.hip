@@ -1,0 +1,289 @@
+// Mixed-precision MFMA GEMMs for CDNA4 (gfx950) with float64 output.
+//
+//  BF16X3: each float64 operand is pre-split into bf16 hi + lo; the kernel
+//          issues hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 and
+//          accumulates in fp32 (relative error ~2^-16 per product).
+//  F32   : operands pre-rounded to fp32, v_mfma_f32_16x16x4_f32.
+//
+// Operands are K-contiguous: A(i,k) = A[i*lda + k], B(k,j) = B[j*ldb + k], with
+// 16-byte aligned rows whose length lda/ldb is padded (with zeros) to a
+// multiple of 8 elements.  One LDS row per tile row holds 64 bytes of K
+// (32 bf16 or 16 fp32); lane (row r = lane%16, group g = lane/16) reads the 16
+// bytes at offset 16*g, which is the bf16 MFMA k-slot layout and, for fp32,
+// the k-permutation "group g owns k = 4g..4g+3 across the 4 sub-steps" shared
+// by A and B.
+#include <hip/hip_bf16.h>
+
+#include "gemm.h"
+#include "prof.h"
+
+namespace mk {
+namespace {
+
+typedef float fx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bfx8 __attribute__((ext_vector_type(8)));
+typedef unsigned int ux4 __attribute__((ext_vector_type(4)));
+
+constexpr int NT = 256;
+constexpr int ROWB = 64;          // bytes of K per LDS row per step
+constexpr int ROWP = ROWB + 16;   // padded LDS row (80 B: conflict-free 16-B reads)
+constexpr int NXCD = 8;
+constexpr int GROUP_M = 8;
+
+enum Mode { M_BF16X3 = 0, M_F32 = 1 };
+
+__device__ __forceinline__ int xcd_remap(int t, int total) {
+  const int q = total / NXCD, r = total % NXCD;
+  const int x = t % NXCD, local = t / NXCD;
+  return x * q + min(x, r) + local;
+}
+
+// Load a (R rows x 64 B) slab of a K-contiguous operand into registers.
+// elem_bytes = 2 (bf16) or 4 (fp32); Klim = padded row length in elements.
+template <int R>
+__device__ __forceinline__ void load_slab(const char* __restrict__ base, int64_t ld_bytes, int r0,
+                                          int64_t kbyte0, int Rlim, int64_t Kbytes,
+                                          ux4 (&reg)[R * ROWB / 16 / NT]) {
+  constexpr int CH = R * ROWB / 16 / NT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * NT;
+    const int r = id / (ROWB / 16);
+    const int q = id % (ROWB / 16);
+    const int gr = r0 + r;
+    const int64_t kb = kbyte0 + q * 16;
+    ux4 v = {0u, 0u, 0u, 0u};
+    if (gr < Rlim && kb < Kbytes) v = *reinterpret_cast<const ux4*>(base + (int64_t)gr * ld_bytes + kb);
+    reg[c] = v;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void store_slab(char* S, const ux4 (&reg)[R * ROWB / 16 / NT]) {
+  constexpr int CH = R * ROWB / 16 / NT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * NT;
+    const int r = id / (ROWB / 16);
+    const int q = id % (ROWB / 16);
+    *reinterpret_cast<ux4*>(S + r * ROWP + q * 16) = reg[c];
+  }
+}
+
+template <int BM, int BN, int MODE>
+__global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
+  constexpr int NPART = MODE == M_BF16X3 ? 2 : 1;   // hi/lo arrays per operand
+  constexpr int EB = MODE == M_BF16X3 ? 2 : 4;      // element bytes
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int CHA = BM * ROWB / 16 / NT;
+  constexpr int CHB = BN * ROWB / 16 / NT;
+
+  __shared__ __attribute__((aligned(16))) char As[2][NPART][BM * ROWP];
+  __shared__ __attribute__((aligned(16))) char Bs[2][NPART][BN * ROWP];
+
+  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
+  int pi = 0;
+  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
+  const GemmProb& P = batch.p[pi];
+
+  const int u = t - P.tile_start;
+  const int per_group = GROUP_M * P.tiles_n;
+  const int g = u / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsize = min(GROUP_M, P.tiles_m - first_m);
+  const int tm = first_m + (u % per_group) % gsize;
+  const int tn = (u % per_group) / gsize;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  fx4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+
+  constexpr int KSTEP = ROWB / EB;  // elements of K per step
+  int nsteps = 0;
+  for (int s = 0; s < P.nseg; ++s) nsteps += (P.seg[s].K + KSTEP - 1) / KSTEP;
+
+  ux4 ra[NPART][CHA], rb[NPART][CHB];
+  auto load = [&](int s, int k0) {
+    const GemmSeg& S = P.seg[s];
+    const int64_t ldab = S.lda * EB, ldbb = S.ldb * EB;
+    const int64_t kb0 = (int64_t)k0 * EB, Kb = (int64_t)S.K * EB;
+    load_slab<BM>(static_cast<const char*>(S.A), ldab, m0, kb0, P.M, Kb, ra[0]);
+    load_slab<BN>(static_cast<const char*>(S.B), ldbb, n0, kb0, P.N, Kb, rb[0]);
+    if (NPART == 2) {
+      load_slab<BM>(static_cast<const char*>(S.Alo), ldab, m0, kb0, P.M, Kb, ra[NPART - 1]);
+      load_slab<BN>(static_cast<const char*>(S.Blo), ldbb, n0, kb0, P.N, Kb, rb[NPART - 1]);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NPART; ++p) {
+      store_slab<BM>(As[buf][p], ra[p]);
+      store_slab<BN>(Bs[buf][p], rb[p]);
+    }
+  };
+
+  int s = 0, k0 = 0;
+  if (nsteps > 0) {
+    load(0, 0);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    int ns = s, nk0 = k0 + KSTEP;
+    if (nk0 >= P.seg[s].K) { ns = s + 1; nk0 = 0; }
+    const bool has_next = step + 1 < nsteps;
+    if (has_next) load(ns, nk0);
+
+    ux4 fa[NPART][FM], fb[NPART][FN];
+#pragma unroll
+    for (int p = 0; p < NPART; ++p) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        fa[p][i] = *reinterpret_cast<const ux4*>(As[buf][p] + (wm0 + i * 16 + li) * ROWP + lg * 16);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        fb[p][j] = *reinterpret_cast<const ux4*>(Bs[buf][p] + (wn0 + j * 16 + li) * ROWP + lg * 16);
+    }
+    if constexpr (MODE == M_BF16X3) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const bfx8 ah = __builtin_bit_cast(bfx8, fa[0][i]);
+          const bfx8 al = __builtin_bit_cast(bfx8, fa[1][i]);
+          const bfx8 bh = __builtin_bit_cast(bfx8, fb[0][j]);
+          const bfx8 bl = __builtin_bit_cast(bfx8, fb[1][j]);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[i][j], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+      for (int sub = 0; sub < 4; ++sub)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            const float a = __builtin_bit_cast(float, fa[0][i][sub]);
+            const float b = __builtin_bit_cast(float, fb[0][j][sub]);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
+          }
+    }
+    if (has_next) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+    s = ns;
+    k0 = nk0;
+  }
+
+  const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn0 + j * 16 + li;
+      if (col >= P.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm0 + i * 16 + lg * 4 + r;
+        if (row >= P.M) continue;
+        double v = (double)acc[i][j][r];
+        if (P.epi == EPI_CB) {
+          const int a = row % P.Dl, bb = row / P.Dl;
+          const int c = col % P.Dr, e = col / P.Dr;
+          double tsum = 0.0;
+          for (int jj = 0; jj < P.d2; ++jj)
+            for (int h = 0; h < P.d1; ++h)
+              tsum += P.cb[bb + P.d1 * (e + P.d2 * (h + P.d1 * jj))] *
+                      P.cbx[a + (int64_t)P.Dl * (h + P.d1 * ((int64_t)c + (int64_t)P.Dr * jj))];
+          v += tsum;
+        }
+        double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
+        v *= alpha;
+        if (beta != 0.0) v += beta * *cp;
+        *cp = v;
+      }
+    }
+  }
+}
+
+template <int BM, int BN, int MODE>
+mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
+  constexpr int EB = MODE == M_BF16X3 ? 2 : 4;
+  GemmBatch b{};
+  int total = 0;
+  for (int i = 0; i < nprob; ++i) {
+    const ProbSpec& ps = probs[i];
+    GemmProb& P = b.p[i];
+    P.M = (int32_t)ps.M;
+    P.N = (int32_t)ps.N;
+    P.nseg = 0;
+    for (int s = 0; s < ps.nseg; ++s) {
+      if (ps.seg[s].K <= 0) continue;
+      const SegSpec& q = ps.seg[s];
+      MK_REQUIRE(q.lda % 8 == 0 && q.ldb % 8 == 0 && q.K <= q.lda && q.K <= q.ldb, MK_BAD_ARG,
+                 "mixed gemm: K-contiguous operands need row length padded to 8 (lda=%lld ldb=%lld K=%lld)",
+                 (long long)q.lda, (long long)q.ldb, (long long)q.K);
+      const void* ptrs[4] = {q.A, q.B, q.Alo, q.Blo};
+      for (int z = 0; z < (MODE == M_BF16X3 ? 4 : 2); ++z)
+        MK_REQUIRE(ptrs[z] != nullptr && (uintptr_t)ptrs[z] % 16 == 0, MK_BAD_ARG,
+                   "mixed gemm: operands must be non-null and 16-byte aligned");
+      GemmSeg& gs = P.seg[P.nseg++];
+      gs.A = q.A; gs.B = q.B; gs.Alo = q.Alo; gs.Blo = q.Blo;
+      gs.lda = q.lda; gs.ldb = q.ldb;
+      // K rounded up to the padded row: the padding is zero
+      gs.K = (int32_t)((q.K + 7) / 8 * 8);
+      gs.vec = 1;
+    }
+    P.tiles_m = (int32_t)cdiv(ps.M, BM);
+    P.tiles_n = (int32_t)cdiv(ps.N, BN);
+    P.tile_start = total;
+    total += P.tiles_m * P.tiles_n;
+    P.C = ps.C; P.ldc = ps.ldc; P.alpha = ps.alpha; P.beta = ps.beta;
+    P.epi = ps.epi; P.Dl = ps.Dl; P.Dr = ps.Dr; P.d1 = ps.d1; P.d2 = ps.d2;
+    P.cb = ps.cb; P.cbx = ps.cbx;
+  }
+  (void)EB;
+  b.nprob = nprob;
+  b.total_tiles = total;
+  if (total == 0) return MK_OK;
+  ProfScope prof(MODE == M_BF16X3 ? "gemm_bf16x3" : "gemm_f32", stream);
+  hipLaunchKernelGGL((gemm_mixed_kernel<BM, BN, MODE>), dim3(total), dim3(NT), 0, stream, b);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+template <int MODE>
+mk_status dispatch(const ProbSpec* probs, int nprob, hipStream_t stream) {
+  MK_REQUIRE(nprob >= 0 && nprob <= MAX_PROB, MK_BAD_ARG, "mixed gemm: too many problems");
+  int64_t tiles128 = 0;
+  for (int i = 0; i < nprob; ++i) {
+    MK_REQUIRE(probs[i].nseg <= MAX_SEG, MK_BAD_ARG, "mixed gemm: too many segments");
+    tiles128 += cdiv(probs[i].M, 128) * cdiv(probs[i].N, 128);
+  }
+  if (tiles128 >= 256) return launch_mixed<128, 128, MODE>(probs, nprob, stream);
+  return launch_mixed<64, 64, MODE>(probs, nprob, stream);
+}
+
+}  // namespace
+
+mk_status gemm_bf16x3(const ProbSpec* probs, int nprob, hipStream_t stream) {
+  return dispatch<M_BF16X3>(probs, nprob, stream);
+}
+mk_status gemm_f32(const ProbSpec* probs, int nprob, hipStream_t stream) {
+  return dispatch<M_F32>(probs, nprob, stream);
+}
+
+}  // namespace mk

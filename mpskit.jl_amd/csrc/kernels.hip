@@ -1,0 +1,557 @@
+// Memory-bound helper kernels; see kernels.h for the contracts.
+#include "kernels.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+
+namespace mk {
+namespace {
+
+constexpr int TPB = 256;
+
+inline dim3 grid1d(int64_t n, int per_thread = 1) {
+  int64_t b = cdiv(n, (int64_t)TPB * per_thread);
+  if (b < 1) b = 1;
+  if (b > (1 << 30)) b = (1 << 30);
+  return dim3((unsigned)b);
+}
+
+#define GRID_STRIDE(i, n) \
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ double env_t(const double* G, int64_t D0, int w, int64_t a, int g,
+                                        int64_t f, int ident) {
+  // environment with channel `ident` replaced by the identity matrix
+  if (g == ident) return a == f ? 1.0 : 0.0;
+  return G[a + D0 * (g + (int64_t)w * f)];
+}
+
+__global__ void prep_L_kernel(const double* __restrict__ GL, const double* __restrict__ W,
+                              const int* __restrict__ klist, int nk, int64_t Da, int64_t Df,
+                              int wl, int d, int wr, int g_ident, double* __restrict__ L) {
+  const int64_t n = Da * d * Df * d * nk;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % Da; t /= Da;
+    const int b = (int)(t % d); t /= d;
+    const int64_t f = t % Df; t /= Df;
+    const int h = (int)(t % d); t /= d;
+    const int kk = (int)t;
+    const int k = klist[kk];
+    double s = 0.0;
+    for (int g = 0; g < wl; ++g) {
+      const double wv = W[g + wl * (b + d * (h + d * k))];
+      if (wv != 0.0) s += env_t(GL, Da, wl, a, g, f, g_ident) * wv;
+    }
+    L[idx] = s;
+  }
+}
+
+__global__ void prep_R_kernel(const double* __restrict__ GR, const double* __restrict__ W,
+                              const int* __restrict__ klist, int nk, int64_t Di, int64_t Dc,
+                              int wl, int d, int wr, int l_ident, double* __restrict__ R) {
+  const int64_t n = Di * d * nk * Dc * d;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t i = t % Di; t /= Di;
+    const int j = (int)(t % d); t /= d;
+    const int kk = (int)(t % nk); t /= nk;
+    const int64_t c = t % Dc; t /= Dc;
+    const int e = (int)t;
+    const int k = klist[kk];
+    double s = 0.0;
+    for (int l = 0; l < wr; ++l) {
+      const double wv = W[k + wl * (e + d * (j + d * l))];
+      if (wv != 0.0) s += wv * env_t(GR, Di, wr, i, l, c, l_ident);
+    }
+    R[idx] = s;
+  }
+}
+
+__global__ void prep_Cp_kernel(const double* __restrict__ GR, const double* __restrict__ W,
+                               int64_t Di, int64_t Dc, int wl, int d, int wr, int l_ident,
+                               double* __restrict__ Cp) {
+  const int64_t n = d * Di * d * Dc;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int h = (int)(t % d); t /= d;
+    const int64_t i = t % Di; t /= Di;
+    const int b = (int)(t % d); t /= d;
+    const int64_t c = t;
+    double s = 0.0;
+    for (int l = 0; l < wr; ++l) {
+      const double wv = W[0 + wl * (b + d * (h + d * l))];
+      if (wv != 0.0) s += wv * env_t(GR, Di, wr, i, l, c, l_ident);
+    }
+    Cp[idx] = s;
+  }
+}
+
+__global__ void prep_CA_kernel(const double* __restrict__ GR, const double* __restrict__ W1,
+                               const double* __restrict__ W2, int64_t Dr, int wl, int d1, int wm,
+                               int d2, int wr, int l_ident, double* __restrict__ CA) {
+  const int64_t n = (int64_t)d1 * Dr * d2 * d1 * Dr * d2;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int h = (int)(t % d1); t /= d1;
+    const int64_t i = t % Dr; t /= Dr;
+    const int j = (int)(t % d2); t /= d2;
+    const int b = (int)(t % d1); t /= d1;
+    const int64_t c = t % Dr; t /= Dr;
+    const int e = (int)t;
+    double s = 0.0;
+    for (int k = 0; k < wm; ++k) {
+      const double w1 = W1[0 + wl * (b + d1 * (h + d1 * k))];
+      if (w1 == 0.0) continue;
+      double r = 0.0;
+      for (int l = 0; l < wr; ++l) {
+        const double w2 = W2[k + wm * (e + d2 * (j + d2 * l))];
+        if (w2 != 0.0) r += w2 * env_t(GR, Dr, wr, i, l, c, l_ident);
+      }
+      s += w1 * r;
+    }
+    CA[idx] = s;
+  }
+}
+
+__global__ void prep_AB_kernel(const double* __restrict__ GL, const double* __restrict__ W1,
+                               const double* __restrict__ W2, int64_t Dl, int wl, int d1, int wm,
+                               int d2, int wr, int g_ident, double* __restrict__ AB) {
+  const int64_t n = Dl * d1 * Dl * d1 * d2 * d2;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % Dl; t /= Dl;
+    const int b = (int)(t % d1); t /= d1;
+    const int64_t f = t % Dl; t /= Dl;
+    const int h = (int)(t % d1); t /= d1;
+    const int e = (int)(t % d2); t /= d2;
+    const int j = (int)t;
+    double s = 0.0;
+    for (int k = 0; k < wm; ++k) {
+      const double w2 = W2[k + wm * (e + d2 * (j + d2 * (wr - 1)))];
+      if (w2 == 0.0) continue;
+      double lsum = 0.0;
+      for (int g = 0; g < wl; ++g) {
+        const double w1 = W1[g + wl * (b + d1 * (h + d1 * k))];
+        if (w1 != 0.0) lsum += env_t(GL, Dl, wl, a, g, f, g_ident) * w1;
+      }
+      s += lsum * w2;
+    }
+    AB[idx] = s;
+  }
+}
+
+__global__ void swap23_kernel(const double* __restrict__ in, int64_t D0, int64_t D1, int64_t D2,
+                              double* __restrict__ out) {
+  const int64_t n = D0 * D1 * D2;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % D0; t /= D0;
+    const int64_t f = t % D2; t /= D2;
+    const int64_t g = t;
+    out[idx] = in[a + D0 * (g + D1 * f)];
+  }
+}
+
+__global__ void env_channels_kernel(const double* __restrict__ src, int64_t Di, int w, int64_t Dc,
+                                    const int* __restrict__ klist, int nk, double* __restrict__ dst) {
+  const int64_t n = Di * nk * Dc;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t i = t % Di; t /= Di;
+    const int k = (int)(t % nk); t /= nk;
+    const int64_t c = t;
+    dst[idx] = src[i + Di * (klist[k] + (int64_t)w * c)];
+  }
+}
+
+__global__ void tl_mid_kernel(const double* __restrict__ t1, const double* __restrict__ O,
+                              int64_t Dm, int wl, int d, int wr, int64_t Dc, double* __restrict__ t2) {
+  const int64_t n = Dm * d * wr * Dc;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t m = t % Dm; t /= Dm;
+    const int k = (int)(t % d); t /= d;
+    const int b = (int)(t % wr); t /= wr;
+    const int64_t c = t;
+    double s = 0.0;
+    for (int h = 0; h < d; ++h)
+      for (int g = 0; g < wl; ++g) {
+        const double ov = O[g + wl * (k + d * (h + d * b))];
+        if (ov != 0.0) s += t1[m + Dm * (g + (int64_t)wl * (h + d * c))] * ov;
+      }
+    t2[idx] = s;
+  }
+}
+
+__global__ void tr_mid_kernel(const double* __restrict__ t1, const double* __restrict__ O,
+                              int64_t Da, int wl, int d, int wr, int64_t Do, double* __restrict__ t2) {
+  const int64_t n = Da * wl * d * Do;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % Da; t /= Da;
+    const int b = (int)(t % wl); t /= wl;
+    const int k = (int)(t % d); t /= d;
+    const int64_t o = t;
+    double s = 0.0;
+    for (int l = 0; l < wr; ++l)
+      for (int h = 0; h < d; ++h) {
+        const double ov = O[b + wl * (k + d * (h + d * l))];
+        if (ov != 0.0) s += t1[a + Da * (h + d * (l + (int64_t)wr * o))] * ov;
+      }
+    t2[idx] = s;
+  }
+}
+
+__global__ void phys2_kernel(const double* __restrict__ P, const double* __restrict__ x, int64_t Dl,
+                             int d1, int64_t Dr, int d2, double alpha, double beta,
+                             double* __restrict__ y) {
+  const int64_t n = Dl * d1 * Dr * d2;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % Dl; t /= Dl;
+    const int b = (int)(t % d1); t /= d1;
+    const int64_t c = t % Dr; t /= Dr;
+    const int e = (int)t;
+    double s = 0.0;
+    for (int j = 0; j < d2; ++j)
+      for (int h = 0; h < d1; ++h)
+        s += P[b + d1 * (e + d2 * (h + d1 * j))] * x[a + Dl * (h + d1 * (c + Dr * j))];
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * y[idx];
+    y[idx] = v;
+  }
+}
+
+__global__ void phys1_kernel(const double* __restrict__ P, const double* __restrict__ x, int64_t Dl,
+                             int d, int64_t Dr, double alpha, double beta, double* __restrict__ y) {
+  const int64_t n = Dl * d * Dr;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % Dl; t /= Dl;
+    const int b = (int)(t % d); t /= d;
+    const int64_t c = t;
+    double s = 0.0;
+    for (int h = 0; h < d; ++h) s += P[b + d * h] * x[a + Dl * (h + (int64_t)d * c)];
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * y[idx];
+    y[idx] = v;
+  }
+}
+
+// ---- reductions -------------------------------------------------------------
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+// lam[g] = sum_{i<D2, j<D1} lvec[i + D2*j] * v[j + D1*(g + w*i)]
+__global__ void reg_lambda_kernel(const double* __restrict__ v, const double* __restrict__ lvec,
+                                  int64_t D1, int w, int64_t D2, double* __restrict__ lam) {
+  __shared__ double sh[TPB / 64];
+  const int g = blockIdx.x;
+  double s = 0.0;
+  const int64_t n = D1 * D2;
+  for (int64_t idx = threadIdx.x; idx < n; idx += blockDim.x) {
+    const int64_t j = idx % D1, i = idx / D1;
+    s += lvec[i + D2 * j] * v[j + D1 * (g + (int64_t)w * i)];
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) lam[g] = s;
+}
+
+__global__ void reg_apply_kernel(double* __restrict__ v, const double* __restrict__ rvec,
+                                 const double* __restrict__ lam, int64_t D1, int w, int64_t D2) {
+  const int64_t n = D1 * w * D2;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % D1; t /= D1;
+    const int g = (int)(t % w); t /= w;
+    const int64_t c = t;
+    v[idx] -= lam[g] * rvec[a + D1 * c];
+  }
+}
+
+constexpr int MD_ITEMS = 16;  // elements of w held per thread in multidot
+
+__global__ void multidot_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                                const double* __restrict__ w, int64_t n, double* __restrict__ partial) {
+  __shared__ double sh[TPB / 64];
+  const int64_t base = (int64_t)blockIdx.x * TPB * MD_ITEMS;
+  double wr[MD_ITEMS];
+#pragma unroll
+  for (int it = 0; it < MD_ITEMS; ++it) {
+    const int64_t i = base + it * TPB + threadIdx.x;
+    wr[it] = i < n ? w[i] : 0.0;
+  }
+  for (int j = 0; j < k; ++j) {
+    const double* Vj = V + (int64_t)j * ldv;
+    double s = 0.0;
+#pragma unroll
+    for (int it = 0; it < MD_ITEMS; ++it) {
+      const int64_t i = base + it * TPB + threadIdx.x;
+      if (i < n) s += Vj[i] * wr[it];
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ void reduce_rows_kernel(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  __shared__ double sh[TPB / 64];
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(int64_t)j * nb + b];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[j] = s;
+}
+
+__global__ void multiaxpy_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                                 const double* __restrict__ c, double* __restrict__ w, int64_t n) {
+  GRID_STRIDE(i, n) {
+    double s = w[i];
+    for (int j = 0; j < k; ++j) s -= c[j] * V[i + (int64_t)j * ldv];
+    w[i] = s;
+  }
+}
+
+__global__ void gemv_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                            const double* __restrict__ s, double* __restrict__ x, int64_t n) {
+  GRID_STRIDE(i, n) {
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc += s[j] * V[i + (int64_t)j * ldv];
+    x[i] = acc;
+  }
+}
+
+__global__ void axpby_kernel(double a, const double* x, double b, double* y,
+                             int64_t n) {
+  GRID_STRIDE(i, n) {
+    double v = a * x[i];
+    if (b != 0.0) v += b * y[i];
+    y[i] = v;
+  }
+}
+
+__global__ void scale_dev_kernel(double* __restrict__ y, const double* __restrict__ sdev, int inv,
+                                 int64_t n) {
+  const double s = inv ? 1.0 / *sdev : *sdev;
+  GRID_STRIDE(i, n) y[i] *= s;
+}
+
+__global__ void nonfinite_kernel(const double* __restrict__ x, int64_t n, int* flag) {
+  int bad = 0;
+  GRID_STRIDE(i, n) {
+    if (!isfinite(x[i])) bad = 1;
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+// ---- precision conversion ---------------------------------------------------
+
+__device__ __forceinline__ void split2(double v, __hip_bfloat16& hi, __hip_bfloat16& lo) {
+  const float f = (float)v;
+  hi = __float2bfloat16(f);
+  const double r = v - (double)__bfloat162float(hi);
+  lo = __float2bfloat16((float)r);
+}
+
+// plain conversion (no transposition): out[c*ldo + r] = M[r + c*ld], zero for rows <= r < ldo
+__global__ void split_plain_kernel(const double* __restrict__ M, int64_t rows, int64_t cols, int64_t ld,
+                                   int64_t ldo, __hip_bfloat16* __restrict__ hi,
+                                   __hip_bfloat16* __restrict__ lo) {
+  const int64_t n = ldo * cols;
+  GRID_STRIDE(idx, n) {
+    const int64_t r = idx % ldo, c = idx / ldo;
+    __hip_bfloat16 h = __float2bfloat16(0.f), l = __float2bfloat16(0.f);
+    if (r < rows) split2(M[r + c * ld], h, l);
+    hi[idx] = h;
+    lo[idx] = l;
+  }
+}
+
+// transposed conversion: out[r*ldo + c] = M[r + c*ld] (zero for cols <= c < ldo), 64x64 LDS tiles
+__global__ void split_trans_kernel(const double* __restrict__ M, int64_t rows, int64_t cols, int64_t ld,
+                                   int64_t ldo, __hip_bfloat16* __restrict__ hi,
+                                   __hip_bfloat16* __restrict__ lo) {
+  __shared__ double tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 4 rows of threads
+  for (int cc = ty; cc < 64; cc += 4) {
+    const int64_t r = r0 + tx, c = c0 + cc;
+    tile[cc][tx] = (r < rows && c < cols) ? M[r + c * ld] : 0.0;
+  }
+  __syncthreads();
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int64_t r = r0 + rr, c = c0 + tx;
+    if (r < rows && c < ldo) {
+      __hip_bfloat16 h, l;
+      split2(tile[tx][rr], h, l);
+      hi[r * ldo + c] = h;
+      lo[r * ldo + c] = l;
+    }
+  }
+}
+
+__global__ void f32_plain_kernel(const double* __restrict__ M, int64_t rows, int64_t cols, int64_t ld,
+                                 int64_t ldo, float* __restrict__ out) {
+  const int64_t n = ldo * cols;
+  GRID_STRIDE(idx, n) {
+    const int64_t r = idx % ldo, c = idx / ldo;
+    out[idx] = r < rows ? (float)M[r + c * ld] : 0.f;
+  }
+}
+
+__global__ void f32_trans_kernel(const double* __restrict__ M, int64_t rows, int64_t cols, int64_t ld,
+                                 int64_t ldo, float* __restrict__ out) {
+  __shared__ double tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int cc = ty; cc < 64; cc += 4) {
+    const int64_t r = r0 + tx, c = c0 + cc;
+    tile[cc][tx] = (r < rows && c < cols) ? M[r + c * ld] : 0.0;
+  }
+  __syncthreads();
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int64_t r = r0 + rr, c = c0 + tx;
+    if (r < rows && c < ldo) out[r * ldo + c] = (float)tile[tx][rr];
+  }
+}
+
+}  // namespace
+
+#define LAUNCH_1D(kern, n, st, ...)                                          \
+  do {                                                                        \
+    if ((n) > 0) {                                                            \
+      hipLaunchKernelGGL(kern, grid1d(n, 4), dim3(TPB), 0, st, __VA_ARGS__);  \
+      MK_CHECK_HIP(hipGetLastError());                                        \
+    }                                                                         \
+    return MK_OK;                                                             \
+  } while (0)
+
+mk_status k_prep_L(const double* GL, const double* W, const int* klist, int nk, int64_t Da,
+                   int64_t Df, int wl, int d, int wr, int g_ident, double* L, hipStream_t st) {
+  LAUNCH_1D(prep_L_kernel, Da * d * Df * d * nk, st, GL, W, klist, nk, Da, Df, wl, d, wr, g_ident, L);
+}
+mk_status k_prep_R(const double* GR, const double* W, const int* klist, int nk, int64_t Di,
+                   int64_t Dc, int wl, int d, int wr, int l_ident, double* R, hipStream_t st) {
+  LAUNCH_1D(prep_R_kernel, Di * d * nk * Dc * d, st, GR, W, klist, nk, Di, Dc, wl, d, wr, l_ident, R);
+}
+mk_status k_prep_Cp(const double* GR, const double* W, int64_t Di, int64_t Dc, int wl, int d,
+                    int wr, int l_ident, double* Cp, hipStream_t st) {
+  LAUNCH_1D(prep_Cp_kernel, (int64_t)d * Di * d * Dc, st, GR, W, Di, Dc, wl, d, wr, l_ident, Cp);
+}
+mk_status k_prep_CA(const double* GR, const double* W1, const double* W2, int64_t Dr, int wl,
+                    int d1, int wm, int d2, int wr, int l_ident, double* CA, hipStream_t st) {
+  LAUNCH_1D(prep_CA_kernel, (int64_t)d1 * Dr * d2 * d1 * Dr * d2, st, GR, W1, W2, Dr, wl, d1, wm, d2,
+            wr, l_ident, CA);
+}
+mk_status k_prep_AB(const double* GL, const double* W1, const double* W2, int64_t Dl, int wl,
+                    int d1, int wm, int d2, int wr, int g_ident, double* AB, hipStream_t st) {
+  LAUNCH_1D(prep_AB_kernel, Dl * d1 * Dl * d1 * d2 * d2, st, GL, W1, W2, Dl, wl, d1, wm, d2, wr,
+            g_ident, AB);
+}
+mk_status k_swap23(const double* in, int64_t D0, int64_t D1, int64_t D2, double* out, hipStream_t st) {
+  LAUNCH_1D(swap23_kernel, D0 * D1 * D2, st, in, D0, D1, D2, out);
+}
+mk_status k_env_channels(const double* src, int64_t Di, int w, int64_t Dc, const int* klist, int nk,
+                         double* dst, hipStream_t st) {
+  LAUNCH_1D(env_channels_kernel, Di * nk * Dc, st, src, Di, w, Dc, klist, nk, dst);
+}
+mk_status k_tl_mid(const double* t1, const double* O, int64_t Dm, int wl, int d, int wr, int64_t Dc,
+                   double* t2, hipStream_t st) {
+  LAUNCH_1D(tl_mid_kernel, Dm * d * wr * Dc, st, t1, O, Dm, wl, d, wr, Dc, t2);
+}
+mk_status k_tr_mid(const double* t1, const double* O, int64_t Da, int wl, int d, int wr, int64_t Do,
+                   double* t2, hipStream_t st) {
+  LAUNCH_1D(tr_mid_kernel, Da * wl * d * Do, st, t1, O, Da, wl, d, wr, Do, t2);
+}
+mk_status k_phys2(const double* P, const double* x, int64_t Dl, int d1, int64_t Dr, int d2,
+                  double alpha, double beta, double* y, hipStream_t st) {
+  LAUNCH_1D(phys2_kernel, Dl * d1 * Dr * d2, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
+}
+mk_status k_phys1(const double* P, const double* x, int64_t Dl, int d, int64_t Dr, double alpha,
+                  double beta, double* y, hipStream_t st) {
+  LAUNCH_1D(phys1_kernel, Dl * d * Dr, st, P, x, Dl, d, Dr, alpha, beta, y);
+}
+
+mk_status k_regularize(double* v, const double* lvec, const double* rvec, int64_t D1, int w, int64_t D2,
+                       double* lam, hipStream_t st) {
+  if (D1 * D2 * w == 0) return MK_OK;
+  hipLaunchKernelGGL(reg_lambda_kernel, dim3(w), dim3(TPB), 0, st, v, lvec, D1, w, D2, lam);
+  MK_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reg_apply_kernel, grid1d(D1 * w * D2, 4), dim3(TPB), 0, st, v, rvec, lam, D1, w, D2);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+int multidot_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, (int64_t)TPB * MD_ITEMS)); }
+
+mk_status k_multidot(const double* V, int64_t ldv, int k, const double* w, int64_t n, double* out,
+                     double* partial, hipStream_t st) {
+  if (k <= 0) return MK_OK;
+  const int nb = multidot_blocks(n);
+  hipLaunchKernelGGL(multidot_kernel, dim3(nb), dim3(TPB), 0, st, V, ldv, k, w, n, partial);
+  MK_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(k), dim3(TPB), 0, st, partial, nb, out);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+mk_status k_multiaxpy(const double* V, int64_t ldv, int k, const double* c, double* w, int64_t n,
+                      hipStream_t st) {
+  if (k <= 0) return MK_OK;
+  LAUNCH_1D(multiaxpy_kernel, n, st, V, ldv, k, c, w, n);
+}
+mk_status k_gemv(const double* V, int64_t ldv, int k, const double* s, double* x, int64_t n,
+                 hipStream_t st) {
+  LAUNCH_1D(gemv_kernel, n, st, V, ldv, k, s, x, n);
+}
+mk_status k_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t st) {
+  LAUNCH_1D(axpby_kernel, n, st, a, x, b, y, n);
+}
+mk_status k_scale_dev(double* y, const double* sdev, int inv, int64_t n, hipStream_t st) {
+  LAUNCH_1D(scale_dev_kernel, n, st, y, sdev, inv, n);
+}
+mk_status k_nonfinite(const double* x, int64_t n, int* flag, hipStream_t st) {
+  LAUNCH_1D(nonfinite_kernel, n, st, x, n, flag);
+}
+
+mk_status k_split_bf16(const double* M, int64_t rows, int64_t cols, int64_t ld, int trans, int64_t ldo,
+                       void* hi, void* lo, hipStream_t st) {
+  if (rows * cols == 0) return MK_OK;
+  auto* h = static_cast<__hip_bfloat16*>(hi);
+  auto* l = static_cast<__hip_bfloat16*>(lo);
+  if (trans) {
+    hipLaunchKernelGGL(split_plain_kernel, grid1d(ldo * cols, 4), dim3(TPB), 0, st, M, rows, cols, ld, ldo, h, l);
+  } else {
+    dim3 g((unsigned)cdiv(rows, 64), (unsigned)cdiv(ldo, 64));
+    hipLaunchKernelGGL(split_trans_kernel, g, dim3(TPB), 0, st, M, rows, cols, ld, ldo, h, l);
+  }
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+mk_status k_to_f32(const double* M, int64_t rows, int64_t cols, int64_t ld, int trans, int64_t ldo,
+                   float* out, hipStream_t st) {
+  if (rows * cols == 0) return MK_OK;
+  if (trans) {
+    hipLaunchKernelGGL(f32_plain_kernel, grid1d(ldo * cols, 4), dim3(TPB), 0, st, M, rows, cols, ld, ldo, out);
+  } else {
+    dim3 g((unsigned)cdiv(rows, 64), (unsigned)cdiv(ldo, 64));
+    hipLaunchKernelGGL(f32_trans_kernel, g, dim3(TPB), 0, st, M, rows, cols, ld, ldo, out);
+  }
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+}  // namespace mk

@@ -1,0 +1,236 @@
+// Lowest-eigenpair Krylov solver for prepared (Hermitian) derivative operators.
+//
+// Replaces fixedpoint(H, x0, :SR, Lanczos(...)) (src/algorithms/fixedpoint.jl:15-18)
+// for the local DMRG / VUMPS eigenproblems.  All orthogonalisation, Ritz
+// residuals and normalisation are float64 on the device; the host only solves
+// the small projected eigenproblem and reads a few scalars per step.
+//
+// Each restart cycle starts from the current vector x with one float64 apply
+// w = H x, which gives the exact Ritz value theta = <x, w> and residual
+// r = w - theta x.  The cycle then extends span{x} by a Krylov space of the
+// (possibly mixed-precision) operator started from r, orthonormalised in
+// float64 (two classical Gram-Schmidt passes), and takes the lowest Ritz pair
+// of the projected matrix whose x-row <q, H x> uses the float64 w.  With a
+// float64 inner operator this is restarted Lanczos; with an FP32 / BF16x3
+// inner operator it is iterative refinement: only the correction is computed
+// in low precision, so every cycle contracts the float64 residual by about the
+// low-precision error and the loop stops on the float64 residual.
+#include <cmath>
+#include <vector>
+
+#include "kernels.h"
+#include "ops.h"
+
+namespace mk {
+namespace {
+
+// Cyclic Jacobi eigen-decomposition of a small dense symmetric matrix
+// (row-major n x n).  Returns eigenvalues in `ev` and eigenvectors as columns
+// of `Q` (row-major).
+void jacobi_eig(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& Q) {
+  Q.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) {
+        const double v = A[(size_t)p * n + q] * A[(size_t)p * n + q];
+        tot += v;
+        if (p != q) off += v;
+      }
+    if (off <= 1e-34 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[(size_t)p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - s * akq;
+          A[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - s * aqk;
+          A[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double qkp = Q[(size_t)k * n + p], qkq = Q[(size_t)k * n + q];
+          Q[(size_t)k * n + p] = c * qkp - s * qkq;
+          Q[(size_t)k * n + q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  ev.resize(n);
+  for (int i = 0; i < n; ++i) ev[i] = A[(size_t)i * n + i];
+}
+
+// lowest eigenpair of the leading k x k block of G (row-major, stride ld)
+void lowest(const std::vector<double>& G, int ld, int k, double& lam, std::vector<double>& s) {
+  std::vector<double> A((size_t)k * k);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) A[(size_t)i * k + j] = 0.5 * (G[(size_t)i * ld + j] + G[(size_t)j * ld + i]);
+  std::vector<double> ev, Q;
+  jacobi_eig(A, k, ev, Q);
+  int imin = 0;
+  for (int i = 1; i < k; ++i)
+    if (ev[i] < ev[imin]) imin = i;
+  lam = ev[imin];
+  s.resize(k);
+  for (int i = 0; i < k; ++i) s[i] = Q[(size_t)i * k + imin];
+}
+
+struct Dev {
+  Ctx* ctx;
+  double* coef;     // device coefficient vector (length >= kmax)
+  double* partial;  // multidot partials
+  // <V_j, u> for j < k, copied to host
+  mk_status dots(const double* V, int64_t n, int k, const double* u, std::vector<double>& out) {
+    MK_TRY(k_multidot(V, n, k, u, n, coef, partial, ctx->stream));
+    out.resize(k);
+    MK_CHECK_HIP(hipMemcpyAsync(out.data(), coef, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
+    MK_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return MK_OK;
+  }
+  // u -= V c with c the device coefficients of the last dots() call
+  mk_status subtract(const double* V, int64_t n, int k, double* u) {
+    return k_multiaxpy(V, n, k, coef, u, n, ctx->stream);
+  }
+  mk_status norm(const double* u, int64_t n, double& nrm) {
+    std::vector<double> t;
+    MK_TRY(dots(u, n, 1, u, t));
+    nrm = std::sqrt(std::max(t[0], 0.0));
+    return MK_OK;
+  }
+};
+
+}  // namespace
+
+// inner Krylov stopping factor for the mixed modes: stop the low-precision
+// cycle once its estimated residual is below eta * (float64 residual at the
+// start of the cycle), i.e. aim for a few digits of contraction per cycle.
+static double inner_eta(int precision) { return precision == MK_FP64 ? 0.0 : 1e-3; }
+
+}  // namespace mk
+
+using namespace mk;
+
+extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter, double tol, int precision,
+                                 double* eigval, double* resnorm, int* numops, int* converged) {
+  MK_REQUIRE(h != nullptr && x != nullptr, MK_BAD_ARG, "eigsolve: null argument");
+  MK_REQUIRE(krylovdim >= 2 && krylovdim <= 512, MK_BAD_ARG, "eigsolve: krylovdim %d not in [2, 512]", krylovdim);
+  MK_REQUIRE(maxiter >= 1, MK_BAD_ARG, "eigsolve: maxiter must be >= 1");
+  MK_REQUIRE(tol >= 0.0, MK_BAD_ARG, "eigsolve: negative tolerance");
+  Op* op = reinterpret_cast<Op*>(h);
+  MK_REQUIRE(op->x_elems == op->y_elems, MK_BAD_SHAPE, "eigsolve: operator is not square");
+  MK_REQUIRE(op->precisions & (1 << precision), MK_BAD_ARG, "eigsolve: precision %d not prepared", precision);
+  Ctx* ctx = op->ctx;
+  MK_CHECK_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t n = op->x_elems;
+  const int m = krylovdim - 1;  // Krylov vectors per cycle besides x
+
+  // basis Q = [x, v_0, ..., v_m] (column-major n x (m + 2)), w = H x, u = work
+  DevBuf Qb, wb, ub, cb, pb;
+  MK_TRY(Qb.alloc((size_t)n * (m + 2) * 8));
+  MK_TRY(wb.alloc((size_t)n * 8));
+  MK_TRY(ub.alloc((size_t)n * 8));
+  MK_TRY(cb.alloc((size_t)(m + 4) * 8));
+  const int64_t nb = std::max<int64_t>(1, cdiv(n, 256 * 16));
+  MK_TRY(pb.alloc((size_t)nb * (m + 4) * 8));
+  double* Q = static_cast<double*>(Qb.ptr);
+  double* w = static_cast<double*>(wb.ptr);
+  double* u = static_cast<double*>(ub.ptr);
+  auto col = [&](int j) { return Q + (int64_t)j * n; };
+  Dev dv{ctx, static_cast<double*>(cb.ptr), static_cast<double*>(pb.ptr)};
+
+  int ops = 0;
+  MK_CHECK_HIP(hipMemcpyAsync(col(0), x, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+  double nrm = 0.0;
+  MK_TRY(dv.norm(col(0), n, nrm));
+  MK_REQUIRE(std::isfinite(nrm), MK_NONFINITE, "eigsolve: start vector is not finite");
+  MK_REQUIRE(nrm > 0.0, MK_BAD_ARG, "eigsolve: zero start vector");
+  MK_TRY(k_axpby(1.0 / nrm, col(0), 0.0, col(0), n, st));
+
+  double theta = 0.0, res = INFINITY;
+  double est_last = INFINITY;
+  bool conv = false;
+  const int ldg = m + 2;
+  std::vector<double> G((size_t)ldg * ldg), c, c2, s;
+  for (int outer = 0;; ++outer) {
+    // float64 mode: accept the Ritz estimate of the previous cycle, as
+    // KrylovKit does, which saves one apply
+    if (precision == MK_FP64 && outer > 0 && est_last <= tol) {
+      res = est_last;
+      conv = true;
+      break;
+    }
+    // float64 Ritz value and residual of x
+    MK_TRY(op_apply(op, col(0), w, 1.0, 0.0, MK_FP64));
+    ++ops;
+    MK_TRY(dv.dots(col(0), n, 1, w, c));
+    theta = c[0];
+    MK_TRY(k_axpby(1.0, w, 0.0, col(1), n, st));
+    MK_TRY(k_axpby(-theta, col(0), 1.0, col(1), n, st));
+    // re-orthogonalise the residual against x (float64)
+    MK_TRY(dv.dots(col(0), n, 1, col(1), c));
+    MK_TRY(dv.subtract(col(0), n, 1, col(1)));
+    MK_TRY(dv.norm(col(1), n, res));
+    MK_REQUIRE(std::isfinite(theta) && std::isfinite(res), MK_NONFINITE, "eigsolve: non-finite Ritz pair");
+    if (res <= tol) {
+      conv = true;
+      break;
+    }
+    if (outer == maxiter) break;
+    MK_TRY(k_axpby(1.0 / res, col(1), 0.0, col(1), n, st));
+
+    std::fill(G.begin(), G.end(), 0.0);
+    G[0] = theta;
+    G[1] = G[ldg] = res;  // <v_0, H x> = |r|
+    int k = 1;            // number of Krylov vectors v_0..v_{k-1} in the cycle
+    const double thresh = std::max(tol, inner_eta(precision) * res);
+    for (int j = 0; j < m; ++j) {
+      // u = H_p v_j
+      MK_TRY(op_apply(op, col(j + 1), u, 1.0, 0.0, precision));
+      ++ops;
+      // two classical Gram-Schmidt passes against [x, v_0..v_j]
+      MK_TRY(dv.dots(Q, n, j + 2, u, c));
+      MK_TRY(dv.subtract(Q, n, j + 2, u));
+      MK_TRY(dv.dots(Q, n, j + 2, u, c2));
+      MK_TRY(dv.subtract(Q, n, j + 2, u));
+      // projected matrix among the Krylov vectors; the x-row stays float64 (<v_i, w>)
+      for (int i = 1; i < j + 2; ++i) G[(size_t)i * ldg + (j + 1)] = c[i] + c2[i];
+      double beta = 0.0;
+      MK_TRY(dv.norm(u, n, beta));
+      MK_REQUIRE(std::isfinite(beta), MK_NONFINITE, "eigsolve: non-finite Krylov vector");
+      double lam;
+      lowest(G, ldg, j + 2, lam, s);
+      est_last = beta * std::fabs(s[j + 1]);
+      k = j + 1;
+      if (est_last <= thresh || beta <= 1e-14 * std::fabs(theta) || j + 1 == m) break;
+      MK_TRY(k_axpby(1.0 / beta, u, 0.0, col(j + 2), n, st));
+      G[(size_t)(j + 2) * ldg + (j + 1)] = beta;
+      std::vector<double> xw;
+      MK_TRY(dv.dots(col(j + 2), n, 1, w, xw));
+      G[(size_t)(j + 2)] = G[(size_t)(j + 2) * ldg] = xw[0];
+    }
+    // lowest Ritz vector of span{x, v_0..v_{k-1}} becomes the new x
+    double lam;
+    lowest(G, ldg, k + 1, lam, s);
+    MK_CHECK_HIP(hipMemcpyAsync(dv.coef, s.data(), (size_t)(k + 1) * 8, hipMemcpyHostToDevice, st));
+    MK_TRY(k_gemv(Q, n, k + 1, dv.coef, u, n, st));
+    MK_TRY(dv.norm(u, n, nrm));
+    MK_TRY(k_axpby(1.0 / nrm, u, 0.0, col(0), n, st));
+    theta = lam;
+  }
+  MK_CHECK_HIP(hipMemcpyAsync(x, col(0), (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+  MK_CHECK_HIP(hipStreamSynchronize(st));
+  if (eigval) *eigval = theta;
+  if (resnorm) *resnorm = res;
+  if (numops) *numops = ops;
+  if (converged) *converged = conv ? 1 : 0;
+  return MK_OK;
+}

@@ -1,0 +1,677 @@
+// Prepared derivative operators: construction (prepare) and application.
+//
+// Generic operators follow the reference's precomputed form
+// (mpo_derivatives.jl:216-288): L = fuse(GL * W1), R = fuse(W2 * GR) and
+// y = L (x R).  Jordan-block operators (hamiltonian_derivatives.jl) use the
+// identity structure of the MPO: with GL's "not started" level and GR's
+// "finished" level equal to the identity, the two-site sum over MPO channels
+// (g, k, l) splits exactly into
+//     k = 0 family   (I x I) x R~_0       -> IC  (includes II, ID)
+//     k = end family L~_end x (I x I)     -> BE  (includes DE, EE)
+//     g = 0, k mid, l = end               -> CB  (physical d^2 x d^2 term)
+//     g = 0, k mid, l mid                 -> CA
+//     g mid, k mid, l = end               -> AB
+//     g mid, k mid, l mid                 -> AA  (connected channels only)
+// and, as prepare_operator!! does (hamiltonian_derivatives.jl:371-468), the
+// k = 0 family and CB fold into CA when CA exists, the k = end family folds
+// into AB when AB exists, and CB folds into AB when only AB exists.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "kernels.h"
+#include "ops.h"
+
+namespace mk {
+
+namespace {
+
+bool okdim(int64_t v) { return v > 0 && v < (1ll << 31); }
+
+// host copy of a small device array
+mk_status fetch(const double* dev, size_t n, std::vector<double>& out) {
+  out.resize(n);
+  MK_CHECK_HIP(hipMemcpy(out.data(), dev, n * 8, hipMemcpyDeviceToHost));
+  for (double v : out)
+    MK_REQUIRE(std::isfinite(v), MK_NONFINITE, "MPO tensor holds a non-finite entry");
+  return MK_OK;
+}
+
+mk_status upload(const void* host, size_t bytes, DevBuf& buf, hipStream_t st) {
+  MK_TRY(buf.alloc(bytes));
+  MK_CHECK_HIP(hipMemcpyAsync(buf.ptr, host, bytes, hipMemcpyHostToDevice, st));
+  MK_CHECK_HIP(hipStreamSynchronize(st));
+  return MK_OK;
+}
+
+struct Wh {  // host view of a dense MPO tensor W[g,b,h,k]
+  std::vector<double> v;
+  int wl = 0, d = 0, wr = 0;
+  double& at(int g, int b, int h, int k) { return v[g + wl * (b + d * (h + d * (size_t)k))]; }
+  double at(int g, int b, int h, int k) const { return v[g + wl * (b + d * (h + d * (size_t)k))]; }
+  bool block_nz(int g, int k) const {
+    for (int b = 0; b < d; ++b)
+      for (int h = 0; h < d; ++h)
+        if (at(g, b, h, k) != 0.0) return true;
+    return false;
+  }
+  bool is_identity(int g, int k) const {
+    for (int b = 0; b < d; ++b)
+      for (int h = 0; h < d; ++h)
+        if (at(g, b, h, k) != (b == h ? 1.0 : 0.0)) return false;
+    return true;
+  }
+  Wh masked(const std::vector<std::vector<bool>>& keep) const {
+    Wh o = *this;
+    for (int g = 0; g < wl; ++g)
+      for (int k = 0; k < wr; ++k)
+        if (!keep[g][k])
+          for (int b = 0; b < d; ++b)
+            for (int h = 0; h < d; ++h) o.at(g, b, h, k) = 0.0;
+    return o;
+  }
+};
+
+std::vector<std::vector<bool>> mask_none(int wl, int wr) {
+  return std::vector<std::vector<bool>>(wl, std::vector<bool>(wr, false));
+}
+
+// Check the Jordan (upper triangular, identity corners) structure of W.
+mk_status check_jordan(const Wh& W, const char* name) {
+  const bool left_boundary = (W.wl == 1), right_boundary = (W.wr == 1);
+  if (!right_boundary)
+    MK_REQUIRE(W.is_identity(0, 0), MK_BAD_ARG, "%s: W[start,start] must be the identity", name);
+  if (!left_boundary)
+    MK_REQUIRE(W.is_identity(W.wl - 1, W.wr - 1), MK_BAD_ARG, "%s: W[end,end] must be the identity", name);
+  // nothing flows back into "not started" or out of "finished"
+  for (int g = 1; g < W.wl && !right_boundary; ++g)
+    MK_REQUIRE(!W.block_nz(g, 0), MK_BAD_ARG, "%s: W[%d, start] must vanish (Jordan form)", name, g);
+  if (!left_boundary)
+    for (int k = 0; k + 1 < W.wr; ++k)
+      MK_REQUIRE(!W.block_nz(W.wl - 1, k), MK_BAD_ARG, "%s: W[end, %d] must vanish (Jordan form)", name, k);
+  return MK_OK;
+}
+
+mk_status make_all_mixed(Op* op) {
+  hipStream_t st = op->ctx->stream;
+  const int pm = op->precisions & (MK_PREC_BF16X3 | MK_PREC_FP32);
+  if (!pm) return MK_OK;
+  OpMat* mats[] = {&op->gen.L, &op->gen.R, &op->IC, &op->BE, &op->CA, &op->Cp, &op->Bp};
+  for (OpMat* m : mats)
+    if (m->present()) MK_TRY(m->make_mixed(pm, st));
+  for (OpMat& m : op->AB) MK_TRY(m.make_mixed(pm, st));
+  MK_CHECK_HIP(hipStreamSynchronize(st));
+  return MK_OK;
+}
+
+// Build a generic "L (x R)" block: L from (GL, W1 masked) over channels klist,
+// R from (W2 masked, GR) over the same channels.
+mk_status build_lxr(Op* op, LxR& blk, const double* GL, const Wh& W1, int64_t Da, int64_t Df, int g_ident,
+                    const double* GR, const Wh& W2, int64_t Di, int64_t Dc, int l_ident,
+                    const std::vector<int>& klist) {
+  hipStream_t st = op->ctx->stream;
+  const int nk = (int)klist.size();
+  const int64_t d1 = W1.d, d2 = W2.d;
+  DevBuf w1d, w2d, kd;
+  MK_TRY(upload(W1.v.data(), W1.v.size() * 8, w1d, st));
+  MK_TRY(upload(W2.v.data(), W2.v.size() * 8, w2d, st));
+  MK_TRY(upload(klist.data(), klist.size() * sizeof(int), kd, st));
+  MK_TRY(blk.L.alloc('A', Da * d1, Df * d1 * nk));
+  MK_TRY(blk.R.alloc('B', Di * d2, nk * Dc * d2));
+  MK_TRY(k_prep_L(GL, static_cast<double*>(w1d.ptr), static_cast<int*>(kd.ptr), nk, Da, Df, W1.wl, (int)d1,
+                  W1.wr, g_ident, blk.L.f64(), st));
+  MK_TRY(k_prep_R(GR, static_cast<double*>(w2d.ptr), static_cast<int*>(kd.ptr), nk, Di, Dc, W2.wl, (int)d2,
+                  W2.wr, l_ident, blk.R.f64(), st));
+  MK_CHECK_HIP(hipStreamSynchronize(st));
+  blk.Mx = Df * d1;
+  blk.Ki = Di * d2;
+  blk.nk = nk;
+  blk.Ny = Dc * d2;
+  blk.Mo = Da * d1;
+  return MK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// applies
+// ---------------------------------------------------------------------------
+
+struct BetaSeq {
+  double beta;
+  bool first = true;
+  double next() {
+    double b = first ? beta : 1.0;
+    first = false;
+    return b;
+  }
+};
+
+mk_status apply_lxr(Op* op, const LxR& blk, const double* x, double* y, double alpha, double beta, int prec) {
+  Ctx* ctx = op->ctx;
+  const int64_t Mx = blk.Mx, Ki = blk.Ki, nk = blk.nk, Ny = blk.Ny, Mo = blk.Mo;
+  const size_t xr_bytes = (size_t)Mx * nk * Ny * 8;
+  if (prec == MK_FP64) {
+    double* xR = nullptr;
+    MK_TRY(carve(ctx, {{xr_bytes, (void**)&xR}}));
+    ProbSpec p1;
+    p1.M = Mx; p1.N = nk * Ny; p1.C = xR; p1.ldc = Mx;
+    p1.add(x, Mx, blk.R.f64(), Ki, Ki);
+    MK_TRY(gemm_f64(&p1, 1, GemmShape{}, ctx->stream));
+    ProbSpec p2;
+    p2.M = Mo; p2.N = Ny; p2.C = y; p2.ldc = Mo; p2.alpha = alpha; p2.beta = beta;
+    p2.add(blk.L.f64(), Mo, xR, Mx * nk, Mx * nk);
+    return gemm_f64(&p2, 1, GemmShape{}, ctx->stream);
+  }
+  const bool bf = prec == MK_BF16X3;
+  double* xR = nullptr;
+  void *mx = nullptr, *mxr = nullptr;
+  MK_TRY(carve(ctx, {{xr_bytes, (void**)&xR},
+                     {view_bytes(Mx, Ki, 'A', prec), &mx},
+                     {view_bytes(Mx * nk, Ny, 'B', prec), &mxr}}));
+  MixedView vx, vxr;
+  MK_TRY(convert_view(ctx, x, Mx, Ki, Mx, 'A', prec, &vx, mx));
+  ProbSpec p1;
+  p1.M = Mx; p1.N = nk * Ny; p1.C = xR; p1.ldc = Mx;
+  if (bf) p1.add(vx.hi, vx.kpad, blk.R.hi.ptr, blk.R.kpad, Ki, vx.lo, blk.R.lo.ptr);
+  else p1.add(vx.hi, vx.kpad, blk.R.f32.ptr, blk.R.kpad, Ki);
+  MK_TRY(run_gemm(&p1, 1, prec, GemmShape{}, ctx->stream));
+  MK_TRY(convert_view(ctx, xR, Mx * nk, Ny, Mx * nk, 'B', prec, &vxr, mxr));
+  ProbSpec p2;
+  p2.M = Mo; p2.N = Ny; p2.C = y; p2.ldc = Mo; p2.alpha = alpha; p2.beta = beta;
+  if (bf) p2.add(blk.L.hi.ptr, blk.L.kpad, vxr.hi, vxr.kpad, Mx * nk, blk.L.lo.ptr, vxr.lo);
+  else p2.add(blk.L.f32.ptr, blk.L.kpad, vxr.hi, vxr.kpad, Mx * nk);
+  return run_gemm(&p2, 1, prec, GemmShape{}, ctx->stream);
+}
+
+// Operand pointers of one GEMM side for precision `prec`.
+struct Side {
+  const void* p = nullptr;
+  const void* lo = nullptr;
+  int64_t ld = 0;
+};
+Side mat_side(const OpMat& m, int prec) {
+  if (prec == MK_FP64) return {m.f64(), nullptr, m.rows};
+  if (prec == MK_BF16X3) return {m.hi.ptr, m.lo.ptr, m.kpad};
+  return {m.f32.ptr, nullptr, m.kpad};
+}
+Side view_side(const MixedView& v) { return {v.hi, v.lo, v.kpad}; }
+
+mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  Ctx* ctx = op->ctx;
+  const int64_t Dl = op->Dl, Dr = op->Dr, d1 = op->d1, d2 = op->d2;
+  const int64_t M = Dl * d1, N = Dr * d2;
+  BetaSeq bs{beta};
+  // mixed-precision views of x: as left factor (M x N) and as right factor (K = M rows)
+  MixedView xa, xb, xca;
+  if (prec != MK_FP64) {
+    void *ma = nullptr, *mb = nullptr, *mc = nullptr;
+    const bool need_a = op->IC.present(), need_b = op->BE.present() || !op->AB.empty();
+    const bool need_ca = op->CA.present();
+    MK_TRY(carve(ctx, {{need_a ? view_bytes(M, N, 'A', prec) : 0, &ma},
+                       {need_b ? view_bytes(M, N, 'B', prec) : 0, &mb},
+                       {need_ca ? view_bytes(Dl, d1 * N, 'A', prec) : 0, &mc}}));
+    if (need_a) MK_TRY(convert_view(ctx, x, M, N, M, 'A', prec, &xa, ma));
+    if (need_b) MK_TRY(convert_view(ctx, x, M, N, M, 'B', prec, &xb, mb));
+    if (need_ca) MK_TRY(convert_view(ctx, x, Dl, d1 * N, Dl, 'A', prec, &xca, mc));
+  }
+  const GemmShape nn{};
+  if (op->IC.present() || op->BE.present()) {
+    ProbSpec p;
+    p.M = M; p.N = N; p.C = y; p.ldc = M; p.alpha = alpha; p.beta = bs.next();
+    if (op->IC.present()) {
+      Side a = prec == MK_FP64 ? Side{x, nullptr, M} : view_side(xa);
+      Side b = mat_side(op->IC, prec);
+      p.add(a.p, a.ld, b.p, b.ld, N, a.lo, b.lo);
+    }
+    if (op->BE.present()) {
+      Side a = mat_side(op->BE, prec);
+      Side b = prec == MK_FP64 ? Side{x, nullptr, M} : view_side(xb);
+      p.add(a.p, a.ld, b.p, b.ld, M, a.lo, b.lo);
+    }
+    if (op->hasP) {
+      p.epi = EPI_CB;
+      p.Dl = (int32_t)Dl; p.Dr = (int32_t)Dr; p.d1 = (int32_t)d1; p.d2 = (int32_t)d2;
+      p.cb = static_cast<const double*>(op->P.ptr);
+      p.cbx = x;
+    }
+    MK_TRY(run_gemm(&p, 1, prec, nn, ctx->stream));
+  }
+  if (op->CA.present()) {
+    // y[a,(b,c,e)] += x[a,(h,i,j)] CA[(h,i,j),(b,c,e)]
+    ProbSpec p;
+    p.M = Dl; p.N = d1 * N; p.C = y; p.ldc = Dl; p.alpha = alpha; p.beta = bs.next();
+    Side a = prec == MK_FP64 ? Side{x, nullptr, Dl} : view_side(xca);
+    Side b = mat_side(op->CA, prec);
+    p.add(a.p, a.ld, b.p, b.ld, d1 * N, a.lo, b.lo);
+    MK_TRY(run_gemm(&p, 1, prec, nn, ctx->stream));
+  }
+  if (!op->AB.empty()) {
+    // y_e[(a,b),c] += sum_j AB_{e,j}[(a,b),(f,h)] x_j[(f,h),c]
+    const double bcur = bs.next();
+    ProbSpec ps[MAX_PROB];
+    for (int e = 0; e < d2; ++e) {
+      ProbSpec& p = ps[e];
+      p.M = M; p.N = Dr; p.C = y + e * M * Dr; p.ldc = M; p.alpha = alpha; p.beta = bcur;
+      for (int j = 0; j < d2; ++j) {
+        Side a = mat_side(op->AB[e + d2 * j], prec);
+        Side b;
+        if (prec == MK_FP64) b = Side{x + j * M * Dr, nullptr, M};
+        else b = Side{static_cast<const char*>(xb.hi) + (size_t)j * Dr * xb.kpad * (prec == MK_BF16X3 ? 2 : 4),
+                      xb.lo ? static_cast<const char*>(xb.lo) + (size_t)j * Dr * xb.kpad * 2 : nullptr, xb.kpad};
+        p.add(a.p, a.ld, b.p, b.ld, M, a.lo, b.lo);
+      }
+    }
+    MK_TRY(run_gemm(ps, (int)d2, prec, nn, ctx->stream));
+  }
+  if (op->gen.present()) MK_TRY(apply_lxr(op, op->gen, x, y, alpha, bs.next(), prec));
+  if (op->hasP && !(op->IC.present() || op->BE.present()))
+    MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, alpha, bs.next(), y,
+                   ctx->stream));
+  if (bs.first) MK_TRY(k_axpby(0.0, y, beta, y, op->y_elems, ctx->stream));
+  return MK_OK;
+}
+
+mk_status apply_ac_jordan(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  Ctx* ctx = op->ctx;
+  const int64_t Dl = op->Dl, Dr = op->Dr, d = op->d1;
+  BetaSeq bs{beta};
+  MixedView xc, xb;
+  if (prec != MK_FP64) {
+    void *mc = nullptr, *mb = nullptr;
+    MK_TRY(carve(ctx, {{view_bytes(Dl, d * Dr, 'A', prec), &mc},
+                       {op->Bp.present() ? view_bytes(Dl * d, Dr, 'B', prec) : 0, &mb}}));
+    MK_TRY(convert_view(ctx, x, Dl, d * Dr, Dl, 'A', prec, &xc, mc));
+    if (op->Bp.present()) MK_TRY(convert_view(ctx, x, Dl * d, Dr, Dl * d, 'B', prec, &xb, mb));
+  }
+  const GemmShape nn{};
+  {
+    // y[a,(b,c)] = x[a,(h,i)] C'[(h,i),(b,c)]
+    ProbSpec p;
+    p.M = Dl; p.N = d * Dr; p.C = y; p.ldc = Dl; p.alpha = alpha; p.beta = bs.next();
+    Side a = prec == MK_FP64 ? Side{x, nullptr, Dl} : view_side(xc);
+    Side b = mat_side(op->Cp, prec);
+    p.add(a.p, a.ld, b.p, b.ld, d * Dr, a.lo, b.lo);
+    MK_TRY(run_gemm(&p, 1, prec, nn, ctx->stream));
+  }
+  if (op->Bp.present()) {
+    // y[(a,b),c] += B'[(a,b),(f,h)] x[(f,h),c]
+    ProbSpec p;
+    p.M = Dl * d; p.N = Dr; p.C = y; p.ldc = Dl * d; p.alpha = alpha; p.beta = bs.next();
+    Side a = mat_side(op->Bp, prec);
+    Side b = prec == MK_FP64 ? Side{x, nullptr, Dl * d} : view_side(xb);
+    p.add(a.p, a.ld, b.p, b.ld, Dl * d, a.lo, b.lo);
+    MK_TRY(run_gemm(&p, 1, prec, nn, ctx->stream));
+  }
+  if (op->gen.present()) MK_TRY(apply_lxr(op, op->gen, x, y, alpha, bs.next(), prec));
+  return MK_OK;
+}
+
+}  // namespace
+
+mk_status op_apply(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  MK_REQUIRE(prec >= 0 && prec <= 2 && (op->precisions & (1 << prec)), MK_BAD_ARG,
+             "apply: precision %d was not prepared (mask %d)", prec, op->precisions);
+  MK_REQUIRE(x != nullptr && y != nullptr, MK_BAD_ARG, "apply: null pointer");
+  MK_REQUIRE(x != y, MK_BAD_ARG, "apply: x and y must not alias");
+  MK_CHECK_HIP(hipSetDevice(op->ctx->device));
+  switch (op->kind) {
+    case OP_C:
+    case OP_AC_GEN:
+    case OP_AC2_GEN:
+      return apply_lxr(op, op->gen, x, y, alpha, beta, prec);
+    case OP_AC2_JOR:
+      return apply_ac2_jordan(op, x, y, alpha, beta, prec);
+    case OP_AC_JOR:
+      return apply_ac_jordan(op, x, y, alpha, beta, prec);
+  }
+  return MK_NOT_SUPPORTED;
+}
+
+}  // namespace mk
+
+using namespace mk;
+
+// ---------------------------------------------------------------------------
+// C ABI: prepares
+// ---------------------------------------------------------------------------
+
+namespace {
+
+mk_status new_op(mk_ctx h, OpKind kind, int precisions, Op** out) {
+  MK_REQUIRE(h != nullptr, MK_BAD_ARG, "null context");
+  MK_REQUIRE(out != nullptr, MK_BAD_ARG, "null output handle");
+  MK_REQUIRE(precisions > 0 && precisions < 8, MK_BAD_ARG, "precisions mask %d", precisions);
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  MK_CHECK_HIP(hipSetDevice(ctx->device));
+  Op* op = new Op();
+  op->ctx = ctx;
+  op->kind = kind;
+  op->precisions = precisions | MK_PREC_FP64;  // FP64 is always available (refinement)
+  *out = op;
+  return MK_OK;
+}
+
+mk_status finish(Op* op, mk_op* out) {
+  mk_status s = make_all_mixed(op);
+  if (s != MK_OK) {
+    delete op;
+    return s;
+  }
+  *out = reinterpret_cast<mk_op>(op);
+  return MK_OK;
+}
+
+#define OP_TRY(expr)            \
+  do {                          \
+    mk_status _s = (expr);      \
+    if (_s != MK_OK) {          \
+      delete op;                \
+      return _s;                \
+    }                           \
+  } while (0)
+
+}  // namespace
+
+extern "C" mk_status mk_prepare_c(mk_ctx h, const double* GL, const double* GR, int64_t Da, int64_t Df,
+                                  int64_t w, int64_t Di, int64_t Dc, int precisions, mk_op* out) {
+  MK_REQUIRE(okdim(Da) && okdim(Df) && okdim(w) && okdim(Di) && okdim(Dc), MK_BAD_SHAPE,
+             "prepare_c: non-positive dimension");
+  MK_REQUIRE(GL && GR, MK_BAD_ARG, "prepare_c: null pointer");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_C, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  LxR& g = op->gen;
+  // L[a,(f,g)] = GL[a,g,f]  (_transpose_tail), R = GR
+  OP_TRY(g.L.alloc('A', Da, Df * w));
+  OP_TRY(k_swap23(GL, Da, w, Df, g.L.f64(), st));
+  OP_TRY(g.R.alloc('B', Di, w * Dc));
+  hipError_t e = hipMemcpyAsync(g.R.f64(), GR, (size_t)Di * w * Dc * 8, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) { delete op; set_error("prepare_c: copy failed"); return MK_HIP_ERROR; }
+  g.Mx = Df; g.Ki = Di; g.nk = w; g.Ny = Dc; g.Mo = Da;
+  op->x_elems = Df * Di;
+  op->y_elems = Da * Dc;
+  op->flops = g.flops();
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_prepare_ac_generic(mk_ctx h, const double* GL, const double* W, const double* GR,
+                                           int64_t Da, int64_t Df, int64_t wl, int64_t d, int64_t wr,
+                                           int64_t Di, int64_t Dc, int precisions, mk_op* out) {
+  MK_REQUIRE(okdim(Da) && okdim(Df) && okdim(wl) && okdim(d) && okdim(wr) && okdim(Di) && okdim(Dc),
+             MK_BAD_SHAPE, "prepare_ac: non-positive dimension");
+  MK_REQUIRE(GL && W && GR, MK_BAD_ARG, "prepare_ac: null pointer");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_AC_GEN, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  Wh Wm;
+  Wm.wl = (int)wl; Wm.d = (int)d; Wm.wr = (int)wr;
+  OP_TRY(fetch(W, (size_t)wl * d * d * wr, Wm.v));
+  LxR& g = op->gen;
+  DevBuf wd, kd;
+  std::vector<int> kl(wr);
+  for (int k = 0; k < wr; ++k) kl[k] = k;
+  OP_TRY(upload(Wm.v.data(), Wm.v.size() * 8, wd, st));
+  OP_TRY(upload(kl.data(), kl.size() * sizeof(int), kd, st));
+  OP_TRY(g.L.alloc('A', Da * d, Df * d * wr));
+  OP_TRY(k_prep_L(GL, static_cast<double*>(wd.ptr), static_cast<int*>(kd.ptr), (int)wr, Da, Df, (int)wl, (int)d,
+                  (int)wr, -1, g.L.f64(), st));
+  OP_TRY(g.R.alloc('B', Di, wr * Dc));
+  hipError_t e = hipMemcpyAsync(g.R.f64(), GR, (size_t)Di * wr * Dc * 8, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) { delete op; set_error("prepare_ac: copy failed"); return MK_HIP_ERROR; }
+  if (hipStreamSynchronize(st) != hipSuccess) { delete op; set_error("prepare_ac: sync failed"); return MK_HIP_ERROR; }
+  g.Mx = Df * d; g.Ki = Di; g.nk = wr; g.Ny = Dc; g.Mo = Da * d;
+  op->x_elems = Df * d * Di;
+  op->y_elems = Da * d * Dc;
+  op->flops = g.flops();
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_prepare_ac2_generic(mk_ctx h, const double* GL, const double* W1, const double* W2,
+                                            const double* GR, int64_t Da, int64_t Df, int64_t wl, int64_t d1,
+                                            int64_t wm, int64_t d2, int64_t wr, int64_t Di, int64_t Dc,
+                                            int precisions, mk_op* out) {
+  MK_REQUIRE(okdim(Da) && okdim(Df) && okdim(wl) && okdim(d1) && okdim(wm) && okdim(d2) && okdim(wr) &&
+                 okdim(Di) && okdim(Dc),
+             MK_BAD_SHAPE, "prepare_ac2: non-positive dimension");
+  MK_REQUIRE(GL && W1 && W2 && GR, MK_BAD_ARG, "prepare_ac2: null pointer");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_AC2_GEN, precisions, &op));
+  Wh A, B;
+  A.wl = (int)wl; A.d = (int)d1; A.wr = (int)wm;
+  B.wl = (int)wm; B.d = (int)d2; B.wr = (int)wr;
+  OP_TRY(fetch(W1, (size_t)wl * d1 * d1 * wm, A.v));
+  OP_TRY(fetch(W2, (size_t)wm * d2 * d2 * wr, B.v));
+  std::vector<int> kl(wm);
+  for (int k = 0; k < wm; ++k) kl[k] = k;
+  OP_TRY(build_lxr(op, op->gen, GL, A, Da, Df, -1, GR, B, Di, Dc, -1, kl));
+  op->x_elems = Df * d1 * Di * d2;
+  op->y_elems = Da * d1 * Dc * d2;
+  op->flops = op->gen.flops();
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_prepare_ac2_jordan(mk_ctx h, const double* GL, const double* W1, const double* W2,
+                                           const double* GR, int64_t Dl, int64_t wl, int64_t d1, int64_t wm,
+                                           int64_t d2, int64_t wr, int64_t Dr, int precisions, mk_op* out) {
+  MK_REQUIRE(okdim(Dl) && okdim(wl) && okdim(d1) && okdim(wm) && okdim(d2) && okdim(wr) && okdim(Dr),
+             MK_BAD_SHAPE, "prepare_ac2_jordan: non-positive dimension");
+  MK_REQUIRE(wm >= 2, MK_BAD_SHAPE, "prepare_ac2_jordan: the shared MPO bond needs >= 2 channels");
+  MK_REQUIRE(d2 <= MAX_PROB && d2 <= MAX_SEG, MK_NOT_SUPPORTED, "prepare_ac2_jordan: d2 > %d", MAX_SEG);
+  MK_REQUIRE(GL && W1 && W2 && GR, MK_BAD_ARG, "prepare_ac2_jordan: null pointer");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_AC2_JOR, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  op->Dl = Dl; op->Dr = Dr; op->d1 = d1; op->d2 = d2;
+  Wh A, B;
+  A.wl = (int)wl; A.d = (int)d1; A.wr = (int)wm;
+  B.wl = (int)wm; B.d = (int)d2; B.wr = (int)wr;
+  OP_TRY(fetch(W1, (size_t)wl * d1 * d1 * wm, A.v));
+  OP_TRY(fetch(W2, (size_t)wm * d2 * d2 * wr, B.v));
+  OP_TRY(check_jordan(A, "W1"));
+  OP_TRY(check_jordan(B, "W2"));
+
+  // channel sets: mid = A-block channels of each bond
+  std::vector<int> midL, midM, midR;
+  if (wl > 1) for (int g = 1; g + 1 < wl; ++g) midL.push_back(g);
+  for (int k = 1; k + 1 < wm; ++k) midM.push_back(k);
+  if (wr > 1) for (int l = 1; l + 1 < wr; ++l) midR.push_back(l);
+  const int kEnd = (int)wm - 1, lEnd = (int)wr - 1;
+
+  auto any_nz = [](const Wh& W, const std::vector<int>& gs, const std::vector<int>& ks) {
+    for (int g : gs) for (int k : ks) if (W.block_nz(g, k)) return true;
+    return false;
+  };
+  const bool C1nz = any_nz(A, {0}, midM);
+  const bool A1nz = any_nz(A, midL, midM);
+  const bool B2nz = any_nz(B, midM, {lEnd});
+  const bool A2nz = any_nz(B, midM, midR);
+  const bool hasCA = C1nz && A2nz;
+  const bool hasAB = A1nz && B2nz;
+
+  DevBuf w1d, w2d;
+  const int64_t M = Dl * d1, N = Dr * d2;
+  double fl = 0.0;
+
+  if (hasCA) {
+    // W1 restricted to row 0 and k != end: k = 0 family + CB + CA
+    auto keep = mask_none((int)wl, (int)wm);
+    for (int k = 0; k < kEnd; ++k) keep[0][k] = true;
+    Wh A1m = A.masked(keep);
+    OP_TRY(upload(A1m.v.data(), A1m.v.size() * 8, w1d, st));
+    OP_TRY(upload(B.v.data(), B.v.size() * 8, w2d, st));
+    OP_TRY(op->CA.alloc('B', d1 * N, d1 * N));
+    OP_TRY(k_prep_CA(GR, static_cast<double*>(w1d.ptr), static_cast<double*>(w2d.ptr), Dr, (int)wl, (int)d1,
+                     (int)wm, (int)d2, (int)wr, lEnd, op->CA.f64(), st));
+    fl += 2.0 * Dl * (double)(d1 * N) * (d1 * N);
+  } else {
+    // IC: k = 0 family, R~ over channel 0 with GR's "finished" level = identity
+    std::vector<int> kl = {0};
+    DevBuf kd;
+    OP_TRY(upload(B.v.data(), B.v.size() * 8, w2d, st));
+    OP_TRY(upload(kl.data(), sizeof(int), kd, st));
+    OP_TRY(op->IC.alloc('B', N, N));
+    OP_TRY(k_prep_R(GR, static_cast<double*>(w2d.ptr), static_cast<int*>(kd.ptr), 1, Dr, Dr, (int)wm, (int)d2,
+                    (int)wr, lEnd, op->IC.f64(), st));
+    OP_TRY(hipStreamSynchronize(st) == hipSuccess ? MK_OK : MK_HIP_ERROR);
+    fl += 2.0 * M * N * N;
+  }
+  if (hasAB) {
+    auto keep = mask_none((int)wl, (int)wm);
+    for (int g = 0; g < wl; ++g) keep[g][kEnd] = true;  // k = end family
+    for (int k : midM) {
+      for (int g : midL) keep[g][k] = true;              // AB
+      if (!hasCA) keep[0][k] = true;                     // CB folds here when CA is absent
+    }
+    Wh A1m = A.masked(keep);
+    OP_TRY(upload(A1m.v.data(), A1m.v.size() * 8, w1d, st));
+    OP_TRY(upload(B.v.data(), B.v.size() * 8, w2d, st));
+    OP_TRY(op->ABall.alloc((size_t)M * M * d2 * d2 * 8));
+    OP_TRY(k_prep_AB(GL, static_cast<double*>(w1d.ptr), static_cast<double*>(w2d.ptr), Dl, (int)wl, (int)d1,
+                     (int)wm, (int)d2, (int)wr, 0, static_cast<double*>(op->ABall.ptr), st));
+    op->AB.resize(d2 * d2);
+    for (int64_t b = 0; b < d2 * d2; ++b)
+      op->AB[b].view('A', static_cast<double*>(op->ABall.ptr) + b * M * M, M, M);
+    fl += 2.0 * d2 * d2 * (double)M * M * Dr;
+  } else if (wl > 1 || A.block_nz(0, kEnd)) {
+    // BE: k = end family, L~ over channel end with GL's "not started" level = identity
+    std::vector<int> kl = {kEnd};
+    DevBuf kd;
+    OP_TRY(upload(A.v.data(), A.v.size() * 8, w1d, st));
+    OP_TRY(upload(kl.data(), sizeof(int), kd, st));
+    OP_TRY(op->BE.alloc('A', M, M));
+    OP_TRY(k_prep_L(GL, static_cast<double*>(w1d.ptr), static_cast<int*>(kd.ptr), 1, Dl, Dl, (int)wl, (int)d1,
+                    (int)wm, 0, op->BE.f64(), st));
+    OP_TRY(hipStreamSynchronize(st) == hipSuccess ? MK_OK : MK_HIP_ERROR);
+    fl += 2.0 * M * M * N;
+  }
+  if (!hasCA && !hasAB) {
+    // CB[b,e,h,j] = sum_{k mid} W1[0,b,h,k] W2[k,e,j,end]
+    std::vector<double> cb(d1 * d2 * d1 * d2, 0.0);
+    bool nz = false;
+    for (int b = 0; b < d1; ++b)
+      for (int e = 0; e < d2; ++e)
+        for (int hh = 0; hh < d1; ++hh)
+          for (int j = 0; j < d2; ++j) {
+            double s = 0.0;
+            for (int k : midM) s += A.at(0, b, hh, k) * B.at(k, e, j, lEnd);
+            cb[b + d1 * (e + d2 * (hh + d1 * j))] = s;
+            nz = nz || s != 0.0;
+          }
+    if (nz) {
+      OP_TRY(upload(cb.data(), cb.size() * 8, op->P, st));
+      op->hasP = true;
+      fl += 2.0 * d1 * d2 * (double)d1 * d2 * Dl * Dr;
+    }
+  }
+  // AA over the connected channels (hamiltonian_derivatives.jl:243-270)
+  if (A1nz && A2nz) {
+    std::vector<int> mids;
+    for (int k : midM) {
+      bool fed = false, feeds = false;
+      for (int g : midL) fed = fed || A.block_nz(g, k);
+      for (int l : midR) feeds = feeds || B.block_nz(k, l);
+      if (fed && feeds) mids.push_back(k);
+    }
+    if (!mids.empty()) {
+      auto keepL = mask_none((int)wl, (int)wm);
+      for (int g : midL) for (int k : mids) keepL[g][k] = true;
+      auto keepR = mask_none((int)wm, (int)wr);
+      for (int k : mids) for (int l : midR) keepR[k][l] = true;
+      OP_TRY(build_lxr(op, op->gen, GL, A.masked(keepL), Dl, Dl, -1, GR, B.masked(keepR), Dr, Dr, -1, mids));
+      fl += op->gen.flops();
+    }
+  }
+  op->x_elems = M * N;
+  op->y_elems = M * N;
+  op->flops = fl;
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_prepare_ac_jordan(mk_ctx h, const double* GL, const double* W, const double* GR,
+                                          int64_t Dl, int64_t wl, int64_t d, int64_t wr, int64_t Dr,
+                                          int precisions, mk_op* out) {
+  MK_REQUIRE(okdim(Dl) && okdim(wl) && okdim(d) && okdim(wr) && okdim(Dr), MK_BAD_SHAPE,
+             "prepare_ac_jordan: non-positive dimension");
+  MK_REQUIRE(GL && W && GR, MK_BAD_ARG, "prepare_ac_jordan: null pointer");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_AC_JOR, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  op->Dl = Dl; op->Dr = Dr; op->d1 = d; op->d2 = 1;
+  Wh Wm;
+  Wm.wl = (int)wl; Wm.d = (int)d; Wm.wr = (int)wr;
+  OP_TRY(fetch(W, (size_t)wl * d * d * wr, Wm.v));
+  OP_TRY(check_jordan(Wm, "W"));
+  const int lEnd = (int)wr - 1;
+  std::vector<int> midL, midR;
+  if (wl > 1) for (int g = 1; g + 1 < wl; ++g) midL.push_back(g);
+  if (wr > 1) for (int l = 1; l + 1 < wr; ++l) midR.push_back(l);
+  double fl = 0.0;
+  DevBuf wd;
+  OP_TRY(upload(Wm.v.data(), Wm.v.size() * 8, wd, st));
+  // C'[(h,i),(b,c)] = sum_l W[0,b,h,l] GR~(i,l,c): row 0 (I, C and D)
+  OP_TRY(op->Cp.alloc('B', d * Dr, d * Dr));
+  OP_TRY(k_prep_Cp(GR, static_cast<double*>(wd.ptr), Dr, Dr, (int)wl, (int)d, (int)wr, lEnd, op->Cp.f64(), st));
+  fl += 2.0 * Dl * (double)(d * Dr) * (d * Dr);
+  if (wl > 1) {
+    // B'[(a,b),(f,h)] = sum_{g>0} GL(a,g,f) W[g,b,h,end]: B and E
+    auto keep = mask_none((int)wl, (int)wr);
+    for (int g = 1; g < wl; ++g) keep[g][lEnd] = true;
+    Wh Wb = Wm.masked(keep);
+    DevBuf wbd, kd;
+    std::vector<int> kl = {lEnd};
+    OP_TRY(upload(Wb.v.data(), Wb.v.size() * 8, wbd, st));
+    OP_TRY(upload(kl.data(), sizeof(int), kd, st));
+    OP_TRY(op->Bp.alloc('A', Dl * d, Dl * d));
+    OP_TRY(k_prep_L(GL, static_cast<double*>(wbd.ptr), static_cast<int*>(kd.ptr), 1, Dl, Dl, (int)wl, (int)d,
+                    (int)wr, 0, op->Bp.f64(), st));
+    OP_TRY(hipStreamSynchronize(st) == hipSuccess ? MK_OK : MK_HIP_ERROR);
+    fl += 2.0 * (double)(Dl * d) * (Dl * d) * Dr;
+  }
+  bool anyA = false;
+  for (int g : midL) for (int l : midR) anyA = anyA || Wm.block_nz(g, l);
+  if (anyA) {
+    // A: generic single-site operator over the middle channels
+    auto keep = mask_none((int)wl, (int)wr);
+    for (int g : midL) for (int l : midR) keep[g][l] = true;
+    Wh Wa = Wm.masked(keep);
+    LxR& gb = op->gen;
+    DevBuf wad, kd;
+    OP_TRY(upload(Wa.v.data(), Wa.v.size() * 8, wad, st));
+    OP_TRY(upload(midR.data(), midR.size() * sizeof(int), kd, st));
+    const int nA = (int)midR.size();
+    OP_TRY(gb.L.alloc('A', Dl * d, Dl * d * nA));
+    OP_TRY(k_prep_L(GL, static_cast<double*>(wad.ptr), static_cast<int*>(kd.ptr), nA, Dl, Dl, (int)wl, (int)d,
+                    (int)wr, -1, gb.L.f64(), st));
+    OP_TRY(gb.R.alloc('B', Dr, nA * Dr));
+    OP_TRY(k_env_channels(GR, Dr, (int)wr, Dr, static_cast<int*>(kd.ptr), nA, gb.R.f64(), st));
+    OP_TRY(hipStreamSynchronize(st) == hipSuccess ? MK_OK : MK_HIP_ERROR);
+    gb.Mx = Dl * d; gb.Ki = Dr; gb.nk = nA; gb.Ny = Dr; gb.Mo = Dl * d;
+    fl += gb.flops();
+  }
+  op->x_elems = Dl * d * Dr;
+  op->y_elems = Dl * d * Dr;
+  op->flops = fl;
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_apply(mk_op h, const double* x, double* y, double alpha, double beta, int precision) {
+  MK_REQUIRE(h != nullptr, MK_BAD_ARG, "apply: null operator");
+  return op_apply(reinterpret_cast<Op*>(h), x, y, alpha, beta, precision);
+}
+
+extern "C" mk_status mk_op_info(mk_op h, int64_t* x_elems, int64_t* y_elems, double* flops) {
+  MK_REQUIRE(h != nullptr, MK_BAD_ARG, "op_info: null operator");
+  Op* op = reinterpret_cast<Op*>(h);
+  if (x_elems) *x_elems = op->x_elems;
+  if (y_elems) *y_elems = op->y_elems;
+  if (flops) *flops = op->flops;
+  return MK_OK;
+}
+
+extern "C" mk_status mk_op_free(mk_op h) {
+  if (!h) return MK_OK;
+  Op* op = reinterpret_cast<Op*>(h);
+  (void)hipSetDevice(op->ctx->device);
+  (void)hipStreamSynchronize(op->ctx->stream);
+  delete op;
+  return MK_OK;
+}

@@ -1,0 +1,102 @@
+// Internal structures of contexts and prepared derivative operators.
+#pragma once
+#include <initializer_list>
+#include <utility>
+#include <vector>
+
+#include "common.h"
+#include "gemm.h"
+
+namespace mk {
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  ~DevBuf() { release(); }
+  mk_status alloc(size_t nbytes);
+  void release();
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf arena;      // stream-ordered scratch for applies / transfers
+  DevBuf smallbuf;   // small stream-ordered scratch (reduction partials, scalars)
+  mk_status scratch(size_t nbytes, void** out);
+  mk_status small(size_t nbytes, void** out);
+};
+
+mk_status carve(Ctx* ctx, std::initializer_list<std::pair<size_t, void**>> regions);
+int64_t pad8(int64_t k);
+
+// An operator matrix used as GEMM operand: role 'A' (rows = M, cols = K) or
+// 'B' (rows = K, cols = N).  float64 column-major, plus optional mixed copies.
+struct OpMat {
+  char role = 'A';
+  int64_t rows = 0, cols = 0;
+  DevBuf own;
+  double* fp = nullptr;
+  DevBuf hi, lo, f32;
+  int64_t kpad = 0;
+  double* f64() const { return fp; }
+  bool present() const { return fp != nullptr; }
+  mk_status alloc(char r, int64_t nr, int64_t nc) {
+    role = r; rows = nr; cols = nc;
+    MK_TRY(own.alloc((size_t)nr * nc * 8));
+    fp = static_cast<double*>(own.ptr);
+    return MK_OK;
+  }
+  void view(char r, double* p, int64_t nr, int64_t nc) { role = r; rows = nr; cols = nc; fp = p; }
+  mk_status make_mixed(int precisions, hipStream_t st);
+};
+
+struct MixedView {
+  void* hi = nullptr;
+  void* lo = nullptr;
+  int64_t kpad = 0;
+};
+
+mk_status convert_view(Ctx* ctx, const double* M, int64_t rows, int64_t cols, int64_t ld, char role,
+                       int prec, MixedView* out, void* mem);
+size_t view_bytes(int64_t rows, int64_t cols, char role, int prec);
+mk_status run_gemm(const ProbSpec* p, int np, int prec, GemmShape sh, hipStream_t st);
+
+enum OpKind { OP_C = 0, OP_AC_GEN = 1, OP_AC2_GEN = 2, OP_AC_JOR = 3, OP_AC2_JOR = 4 };
+
+// Two-step "L (x R)" block: xR = x (Mx x Ki) * R (Ki x nk*Ny);  y = L (Mo x Mx*nk) * xR.
+struct LxR {
+  OpMat L, R;
+  int64_t Mx = 0, Ki = 0, nk = 0, Ny = 0, Mo = 0;
+  bool present() const { return L.present(); }
+  double flops() const { return 2.0 * Mx * Ki * nk * Ny + 2.0 * Mo * Mx * nk * Ny; }
+};
+
+struct Op {
+  Ctx* ctx = nullptr;
+  OpKind kind = OP_C;
+  int precisions = MK_PREC_FP64;
+  int64_t Dl = 0, Dr = 0, d1 = 0, d2 = 0;  // Jordan dims (bra == ket)
+  int64_t x_elems = 0, y_elems = 0;
+  double flops = 0.0;
+
+  // generic operators and the A/AA parts of Jordan operators
+  LxR gen;
+
+  // Jordan two-site terms
+  OpMat IC, BE, CA;
+  DevBuf ABall;
+  std::vector<OpMat> AB;  // (e, j) blocks at index e + d2*j
+  DevBuf P;               // two-site (or one-site) physical term, if not folded
+  bool hasP = false;
+
+  // Jordan one-site terms
+  OpMat Cp, Bp;
+};
+
+mk_status op_apply(Op* op, const double* x, double* y, double alpha, double beta, int prec);
+
+}  // namespace mk

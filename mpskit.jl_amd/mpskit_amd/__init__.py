@@ -1,0 +1,25 @@
+"""MI355X-native effective-Hamiltonian / transfer-matrix path for MPSKit.jl.
+
+Host-side mirror of MPSKit.jl's derivative-operator, transfer-matrix and
+environment interface over the C ABI of ``libmpskit_hip.so`` (HIP/CDNA4
+kernels for gfx950).  Importing this package does not touch the GPU; the first
+call that needs the library loads it and raises if it is missing.
+"""
+from . import models, tensors
+from ._lib import BF16X3, FP32, FP64, EXPORTED, LIB_PATH, MKError, NotConvergedError, SpaceMismatch, load
+from .derivatives import (
+    DerivativeOperator, JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, MPO_AC2_Hamiltonian,
+    MPO_AC_Hamiltonian, MPO_C_Hamiltonian, MPODerivativeOperator, PreparedDerivative, prepare_operator,
+)
+from .transfer import (
+    ProductTransferMatrix, RegTransferMatrix, SingleTransferMatrix, TransferMatrix, regularize, regularize_,
+    transfer_left, transfer_right,
+)
+
+__all__ = [
+    "models", "tensors", "BF16X3", "FP32", "FP64", "EXPORTED", "LIB_PATH", "MKError", "NotConvergedError",
+    "SpaceMismatch", "load", "DerivativeOperator", "JordanMPO_AC2_Hamiltonian", "JordanMPO_AC_Hamiltonian",
+    "MPO_AC2_Hamiltonian", "MPO_AC_Hamiltonian", "MPO_C_Hamiltonian", "MPODerivativeOperator",
+    "PreparedDerivative", "prepare_operator", "ProductTransferMatrix", "RegTransferMatrix",
+    "SingleTransferMatrix", "TransferMatrix", "regularize", "regularize_", "transfer_left", "transfer_right",
+]

@@ -1,0 +1,239 @@
+"""Effective-Hamiltonian (derivative) operators on the HIP path.
+
+Mirrors MPSKit's operator family (src/algorithms/derivatives/):
+
+  DerivativeOperator           derivatives.jl:8-11      ``H(x)``, ``H * x``
+  MPO_C_Hamiltonian            mpo_derivatives.jl:23-26
+  MPO_AC_Hamiltonian           mpo_derivatives.jl:28-31
+  MPO_AC2_Hamiltonian          mpo_derivatives.jl:33-36
+  PrecomputedDerivative        mpo_derivatives.jl:168-188 (prepared form)
+  JordanMPO_AC_Hamiltonian     hamiltonian_derivatives.jl:15-107
+  JordanMPO_AC2_Hamiltonian    hamiltonian_derivatives.jl:168-369
+  prepare_operator             prepare_operator!! (derivatives.jl:236-246)
+
+Every operator is prepared once into an immutable ``mk_op`` handle; applying
+it enqueues the HIP kernels on torch's current stream.  Inputs/outputs are
+column-major float64 GPU tensors in the reference's leg order (see
+``tensors``).  ``precision`` selects the bulk arithmetic of an apply:
+"fp64" (reference arithmetic), "fp32" (FP32 MFMA) or "bf16x3" (split-BF16
+MFMA with FP32 accumulation); inputs and outputs are float64 in every mode.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable
+
+import torch
+
+from . import tensors as T
+from ._lib import BF16X3, FP32, FP64, PREC_NAMES, SpaceMismatch, check, load
+from .context import default_context
+
+PrecisionArg = str | int
+
+
+def _prec(p: PrecisionArg) -> int:
+    if isinstance(p, str):
+        return PREC_NAMES[p.lower()]
+    return int(p)
+
+
+def _mask(precisions: Iterable[PrecisionArg] | PrecisionArg | None) -> int:
+    if precisions is None:
+        return 1 << FP64
+    if isinstance(precisions, (str, int)):
+        precisions = [precisions]
+    m = 1 << FP64
+    for p in precisions:
+        m |= 1 << _prec(p)
+    return m
+
+
+def _req(t: torch.Tensor, shape, name: str) -> torch.Tensor:
+    if tuple(t.shape) != tuple(shape):
+        raise SpaceMismatch(1, f"{name}: shape {tuple(t.shape)} != expected {tuple(shape)}")
+    return T.as_colmajor(t)
+
+
+class DerivativeOperator:
+    """Abstract supertype (derivatives.jl:8-11): callable, and ``H * x == H(x)``."""
+
+    def __call__(self, x, *args, **kw):  # pragma: no cover - abstract
+        raise NotImplementedError
+
+    def __mul__(self, x):
+        return self(x)
+
+
+class PreparedDerivative(DerivativeOperator):
+    """A prepared operator handle (``PrecomputedDerivative`` / prepared Jordan operator)."""
+
+    def __init__(self, handle: C.c_void_p, ctx, x_shape, y_shape, kind: str, precisions: int):
+        self.handle = handle
+        self.ctx = ctx
+        self.x_shape = tuple(int(n) for n in x_shape)
+        self.y_shape = tuple(int(n) for n in y_shape)
+        self.kind = kind
+        self.precisions = precisions
+        xe, ye, fl = C.c_int64(), C.c_int64(), C.c_double()
+        check(load().mk_op_info(handle, C.byref(xe), C.byref(ye), C.byref(fl)))
+        self.flops = fl.value
+
+    def __call__(self, x: torch.Tensor, precision: PrecisionArg = "fp64", out: torch.Tensor | None = None,
+                 alpha: float = 1.0, beta: float = 0.0) -> torch.Tensor:
+        x = _req(x, self.x_shape, "x")
+        if out is None:
+            out = T.empty(self.y_shape, x.device)
+            beta = 0.0
+        else:
+            if tuple(out.shape) != self.y_shape or not T.is_colmajor(out):
+                raise SpaceMismatch(1, "out: wrong shape or layout")
+        self.ctx.sync_stream()
+        check(load().mk_apply(self.handle, T.ptr(x), T.ptr(out), float(alpha), float(beta), _prec(precision)))
+        return out
+
+    def eigsolve(self, x0: torch.Tensor, krylovdim: int = 30, maxiter: int = 100, tol: float = 1e-12,
+                 precision: PrecisionArg = "fp64"):
+        """Lowest (:SR) eigenpair, ``fixedpoint(H, x0, :SR, Lanczos(...))`` (fixedpoint.jl:15-18).
+
+        Returns (eigenvalue, eigenvector, info) with info = dict(normres, numops, converged).
+        """
+        x = _req(x0, self.x_shape, "x0").clone()
+        self.ctx.sync_stream()
+        ev, rn, nops, conv = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        check(load().mk_eigsolve(self.handle, T.ptr(x), int(krylovdim), int(maxiter), float(tol),
+                                 _prec(precision), C.byref(ev), C.byref(rn), C.byref(nops), C.byref(conv)))
+        return ev.value, x, dict(normres=rn.value, numops=nops.value, converged=bool(conv.value))
+
+    def free(self) -> None:
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            load().mk_op_free(h)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Generic MPO derivatives (mpo_derivatives.jl)
+# ---------------------------------------------------------------------------
+
+class MPODerivativeOperator(DerivativeOperator):
+    """``MPODerivativeOperator`` (mpo_derivatives.jl:10-16): environments plus the
+    MPO tensors in between.  Its raw apply prepares on first use and reuses the
+    prepared form (the same contraction, associated as L * (x * R))."""
+
+    def __init__(self, leftenv, operators, rightenv):
+        self.leftenv = leftenv
+        self.operators = tuple(operators)
+        self.rightenv = rightenv
+        self._prepared = None
+
+    def __call__(self, x, precision: PrecisionArg = "fp64", **kw):
+        if self._prepared is None:
+            self._prepared = prepare_operator(self)
+        return self._prepared(x, precision, **kw)
+
+
+def MPO_C_Hamiltonian(GL, GR):
+    return MPODerivativeOperator(GL, (), GR)
+
+
+def MPO_AC_Hamiltonian(GL, W, GR):
+    return MPODerivativeOperator(GL, (W,), GR)
+
+
+def MPO_AC2_Hamiltonian(GL, W1, W2, GR):
+    return MPODerivativeOperator(GL, (W1, W2), GR)
+
+
+class JordanMPO_AC_Hamiltonian(DerivativeOperator):
+    """``JordanMPO_AC_Hamiltonian`` (hamiltonian_derivatives.jl:15-107): bra == ket."""
+
+    def __init__(self, GL, W, GR):
+        self.GL, self.W, self.GR = GL, W, GR
+        self._prepared = None
+
+    def __call__(self, x, precision: PrecisionArg = "fp64", **kw):
+        if self._prepared is None:
+            self._prepared = prepare_operator(self)
+        return self._prepared(x, precision, **kw)
+
+
+class JordanMPO_AC2_Hamiltonian(DerivativeOperator):
+    """``JordanMPO_AC2_Hamiltonian`` (hamiltonian_derivatives.jl:168-369): bra == ket."""
+
+    def __init__(self, GL, W1, W2, GR):
+        self.GL, self.W1, self.W2, self.GR = GL, W1, W2, GR
+        self._prepared = None
+
+    def __call__(self, x, precision: PrecisionArg = "fp64", **kw):
+        if self._prepared is None:
+            self._prepared = prepare_operator(self)
+        return self._prepared(x, precision, **kw)
+
+
+def prepare_operator(H, precisions=None) -> PreparedDerivative:
+    """``prepare_operator!!`` (derivatives.jl:236-246, mpo_derivatives.jl:216-247,
+    hamiltonian_derivatives.jl:109-157, :371-468)."""
+    if isinstance(H, PreparedDerivative):
+        return H
+    lib = load()
+    ctx = default_context()
+    mask = _mask(precisions)
+    h = C.c_void_p()
+    if isinstance(H, MPODerivativeOperator):
+        GL, GR = H.leftenv, H.rightenv
+        if GL.dim() != 3 or GR.dim() != 3:
+            raise SpaceMismatch(1, "environments must be D x w x D tensors")
+        Da, wl, Df = GL.shape
+        Di, wr, Dc = GR.shape
+        GL, GR = T.as_colmajor(GL), T.as_colmajor(GR)
+        n = len(H.operators)
+        if n == 0:
+            if wl != wr:
+                raise SpaceMismatch(1, "C derivative: MPO bond of GL and GR differ")
+            check(lib.mk_prepare_c(ctx.handle, T.ptr(GL), T.ptr(GR), Da, Df, wl, Di, Dc, mask, C.byref(h)))
+            return PreparedDerivative(h, ctx, (Df, Di), (Da, Dc), "C", mask)
+        if n == 1:
+            W = _req(H.operators[0], (wl, H.operators[0].shape[1], H.operators[0].shape[1], wr), "W")
+            d = W.shape[1]
+            check(lib.mk_prepare_ac_generic(ctx.handle, T.ptr(GL), T.ptr(W), T.ptr(GR), Da, Df, wl, d, wr, Di, Dc,
+                                            mask, C.byref(h)))
+            return PreparedDerivative(h, ctx, (Df, d, Di), (Da, d, Dc), "AC", mask)
+        if n == 2:
+            W1, W2 = H.operators
+            wm = W1.shape[3]
+            d1, d2 = W1.shape[1], W2.shape[1]
+            W1 = _req(W1, (wl, d1, d1, wm), "W1")
+            W2 = _req(W2, (wm, d2, d2, wr), "W2")
+            check(lib.mk_prepare_ac2_generic(ctx.handle, T.ptr(GL), T.ptr(W1), T.ptr(W2), T.ptr(GR), Da, Df, wl, d1,
+                                             wm, d2, wr, Di, Dc, mask, C.byref(h)))
+            return PreparedDerivative(h, ctx, (Df, d1, Di, d2), (Da, d1, Dc, d2), "AC2", mask)
+        raise ValueError("MPODerivativeOperator with more than two MPO tensors")
+    if isinstance(H, JordanMPO_AC_Hamiltonian):
+        Dl, wl, _ = H.GL.shape
+        Dr, wr, _ = H.GR.shape
+        d = H.W.shape[1]
+        GL = _req(H.GL, (Dl, wl, Dl), "GL")
+        GR = _req(H.GR, (Dr, wr, Dr), "GR")
+        W = _req(H.W, (wl, d, d, wr), "W")
+        check(lib.mk_prepare_ac_jordan(ctx.handle, T.ptr(GL), T.ptr(W), T.ptr(GR), Dl, wl, d, wr, Dr, mask,
+                                       C.byref(h)))
+        return PreparedDerivative(h, ctx, (Dl, d, Dr), (Dl, d, Dr), "JordanAC", mask)
+    if isinstance(H, JordanMPO_AC2_Hamiltonian):
+        Dl, wl, _ = H.GL.shape
+        Dr, wr, _ = H.GR.shape
+        wm, d1, d2 = H.W1.shape[3], H.W1.shape[1], H.W2.shape[1]
+        GL = _req(H.GL, (Dl, wl, Dl), "GL")
+        GR = _req(H.GR, (Dr, wr, Dr), "GR")
+        W1 = _req(H.W1, (wl, d1, d1, wm), "W1")
+        W2 = _req(H.W2, (wm, d2, d2, wr), "W2")
+        check(lib.mk_prepare_ac2_jordan(ctx.handle, T.ptr(GL), T.ptr(W1), T.ptr(W2), T.ptr(GR), Dl, wl, d1, wm, d2,
+                                        wr, Dr, mask, C.byref(h)))
+        return PreparedDerivative(h, ctx, (Dl, d1, Dr, d2), (Dl, d1, Dr, d2), "JordanAC2", mask)
+    raise TypeError(f"cannot prepare {type(H).__name__}")

@@ -1,0 +1,186 @@
+"""Brute-force pins for the oracle: dense Hamiltonians, small MPS, closed forms.
+
+TEST INFRASTRUCTURE ONLY (same rule as ``oracle.oracle``): used by ``tests/``
+and ``__graft_entry__.smoke()`` as the checker, never by the product path.
+
+Everything here is built without the index strings of ``oracle.oracle``:
+Hamiltonians are sums of Kronecker products of spin matrices, states are
+dense vectors contracted from MPS tensors site by site, and projected
+operators are ``P_bra^T H P_ket`` with explicit isometries.  A dropped term,
+a wrong sign or a transposed leg in any contraction restated in
+``oracle.oracle`` changes ``P^T H P x`` and fails the comparison.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+# spin-1/2 operators (real): S^z, S^+, S^-, and Pauli X, Z
+SZ = np.array([[0.5, 0.0], [0.0, -0.5]])
+SP = np.array([[0.0, 1.0], [0.0, 0.0]])
+SM = SP.T.copy()
+SX = 0.5 * (SP + SM)
+PX = 2.0 * SX
+PZ = 2.0 * SZ
+I2 = np.eye(2)
+
+
+def _site_op(op, i, L):
+    """Dense-sparse embedding of a one-site operator (site 0 is most significant)."""
+    m = sp.identity(1, format="csr")
+    for j in range(L):
+        m = sp.kron(m, sp.csr_matrix(op) if j == i else sp.identity(op.shape[0]), format="csr")
+    return m
+
+
+def bond_sum(L, terms, dist=1):
+    """sum_i sum_(coef, a, b) coef * a_i b_{i+dist} for an open chain."""
+    H = sp.csr_matrix((2 ** L, 2 ** L))
+    for i in range(L - dist):
+        for coef, a, b in terms:
+            H = H + coef * (_site_op(a, i, L) @ _site_op(b, i + dist, L))
+    return H
+
+
+def field_sum(L, coef, op):
+    H = sp.csr_matrix((2 ** L, 2 ** L))
+    for i in range(L):
+        H = H + coef * _site_op(op, i, L)
+    return H
+
+
+def dense_tfim(L, g):
+    """H = -sum sigma^z sigma^z - g sum sigma^x (test/setup/testsetup.jl:102-119)."""
+    return bond_sum(L, [(-1.0, PZ, PZ)]) + field_sum(L, -g, PX)
+
+
+def dense_xxz(L, delta):
+    """H = sum S^x S^x + S^y S^y + delta S^z S^z = sum (S+S- + S-S+)/2 + delta SzSz."""
+    return bond_sum(L, [(0.5, SP, SM), (0.5, SM, SP), (delta, SZ, SZ)])
+
+
+def dense_j1j2(L, j1, j2):
+    t = [(0.5, SP, SM), (0.5, SM, SP), (1.0, SZ, SZ)]
+    return j1 * bond_sum(L, t, 1) + j2 * bond_sum(L, t, 2)
+
+
+def mpo_to_dense(Ws):
+    """Contract a finite MPO (W[g,b,h,k], boundary bonds of dimension 1) to a matrix."""
+    T = Ws[0][0]                                  # (b, h, k)
+    T = T.reshape(T.shape[0], T.shape[1], T.shape[2])
+    for W in Ws[1:]:
+        # T[B, H, g] W[g, b, h, k] -> T[(B b), (H h), k]
+        nb, nh = T.shape[0], T.shape[1]
+        T = np.einsum("BHg,gbhk->BbHhk", T, W).reshape(nb * W.shape[1], nh * W.shape[2], W.shape[3])
+    return T[:, :, 0]
+
+
+# ---------------------------------------------------------------------------
+# Small MPS utilities
+# ---------------------------------------------------------------------------
+
+def random_mps(L, d, Dmax, rng):
+    """Random open-boundary MPS tensors A[l,p,r] with bonds min(d^k, d^(L-k), Dmax)."""
+    dims = [min(d ** k, d ** (L - k), Dmax) for k in range(L + 1)]
+    return [rng.standard_normal((dims[k], d, dims[k + 1])) for k in range(L)]
+
+
+def left_canonical(As):
+    """QR sweep left->right; returns (ALs, norm) with the state normalised."""
+    out, R = [], np.eye(1)
+    for A in As:
+        A = np.einsum("lm,mpr->lpr", R, A)
+        l, p, r = A.shape
+        Q, R = np.linalg.qr(A.reshape(l * p, r, order="F"))
+        k = Q.shape[1]
+        out.append(Q.reshape(l, p, k, order="F"))
+    nrm = R[0, 0]
+    out[-1] = out[-1] * np.sign(nrm)
+    return out
+
+
+def right_canonical(As):
+    out, R = [], np.eye(1)
+    for A in reversed(As):
+        A = np.einsum("lpm,mr->lpr", A, R)
+        l, p, r = A.shape
+        Q, Rt = np.linalg.qr(A.reshape(l, p * r, order="F").T)
+        k = Q.shape[1]
+        out.append(Q.T.reshape(k, p, r, order="F"))
+        R = Rt.T
+    out.reverse()
+    out[0] = out[0] * np.sign(R[0, 0])
+    return out
+
+
+def mps_to_dense(As):
+    psi = As[0][0]                                 # (p, r)
+    for A in As[1:]:
+        psi = np.einsum("Pm,mpr->Ppr", psi, A).reshape(-1, A.shape[2])
+    return psi[:, 0]
+
+
+def left_block(ALs):
+    """Matrix U[(s_1..s_k), r] of the left block (site 1 most significant)."""
+    U = np.ones((1, 1))
+    for A in ALs:
+        U = np.einsum("Sl,lpr->Spr", U, A).reshape(-1, A.shape[2])
+    return U
+
+
+def right_block(ARs):
+    """Matrix V[l, (s_k..s_L)] of the right block."""
+    V = np.ones((1, 1))
+    for A in reversed(ARs):
+        V = np.einsum("lpr,rS->lpS", A, V).reshape(A.shape[0], -1)
+    return V
+
+
+def _fortran_cols(T, nrow_axes):
+    """Rows: C-order over the first ``nrow_axes`` axes; columns: Fortran order
+    over the rest (first column axis fastest), matching the library layout."""
+    nrow = int(np.prod(T.shape[:nrow_axes]))
+    ncol_axes = T.ndim - nrow_axes
+    perm = list(range(nrow_axes)) + list(range(T.ndim - 1, nrow_axes - 1, -1))
+    return T.transpose(perm).reshape(nrow, -1)
+
+
+def isometry_ac2(ALs, ARs, j, d):
+    """P[(s), (f,h,i,jj)] for the two-site centre at sites j, j+1 (0-based)."""
+    U = left_block(ALs[:j])          # (S_l, f)
+    V = right_block(ARs[j + 2:])     # (i, S_r)
+    T = np.einsum("Sf,hH,jJ,iR->ShjRfHiJ", U, np.eye(d), np.eye(d), V)
+    return _fortran_cols(T, 4)
+
+
+def isometry_ac(ALs, ARs, j, d):
+    """P[(s), (f,h,i)] for the one-site centre at site j (0-based)."""
+    U = left_block(ALs[:j])
+    V = right_block(ARs[j + 1:])
+    T = np.einsum("Sf,hH,iR->ShRfHi", U, np.eye(d), V)
+    return _fortran_cols(T, 3)
+
+
+def isometry_c(ALs, ARs, j):
+    """P[(s), (f,i)] for the bond left of site j (0-based): left block = sites < j."""
+    U = left_block(ALs[:j])
+    V = right_block(ARs[j:])
+    T = np.einsum("Sf,iR->SRfi", U, V)
+    return _fortran_cols(T, 2)
+
+
+# ---------------------------------------------------------------------------
+# Closed-form energies
+# ---------------------------------------------------------------------------
+
+def tfim_open_energy(L, g):
+    """Open-chain TFIM ground energy -sum_k s_k(M), M bidiagonal (M_ii=g, M_i,i+1=1)
+    (Lieb-Schultz-Mattis / Pfeuty)."""
+    M = np.diag(np.full(L, float(g))) + np.diag(np.ones(L - 1), 1)
+    return -np.linalg.svd(M, compute_uv=False).sum()
+
+
+TFIM_INFINITE_G1 = -4.0 / np.pi
+HEISENBERG_INFINITE = 0.25 - np.log(2.0)
+MAJUMDAR_GHOSH = -0.375
+HEISENBERG_L20_OBC = -8.6824724   # test/algorithms/groundstate.jl:172-174 (atol 1e-4)

@@ -1,0 +1,201 @@
+"""Pins of the CPU oracle (no GPU).
+
+Every oracle routine is checked against something that does not share its
+index strings: dense Hamiltonians built from Kronecker products, projected on
+random mixed-canonical MPS (bra != ket for the generic operators), closed-form
+energies, and algebraic identities.  A planted-mistake test shows the pins
+catch a transposed leg.
+"""
+import numpy as np
+import pytest
+
+from oracle import brute as B
+from oracle import oracle as O
+from mpskit_amd import models as M
+
+L = 7
+CHAINS = {
+    "tfim": (lambda: M.tfim(L, 0.7), lambda: B.dense_tfim(L, 0.7)),
+    "xxz": (lambda: M.xxz(L, 0.6), lambda: B.dense_xxz(L, 0.6)),
+    "j1j2": (lambda: M.j1j2(L, 1.0, 0.3), lambda: B.dense_j1j2(L, 1.0, 0.3)),
+}
+
+
+def envs(Ws, ket, bra):
+    ALk, ARk = B.left_canonical(ket), B.right_canonical(ket)
+    ALb, ARb = B.left_canonical(bra), B.right_canonical(bra)
+    GL = [np.ones((1, 1, 1))]
+    for j in range(len(Ws)):
+        GL.append(O.transfer_left_mpo(GL[-1], Ws[j], ALk[j], ALb[j]))
+    GR = [None] * (len(Ws) + 1)
+    GR[-1] = np.ones((1, 1, 1))
+    for j in range(len(Ws) - 1, -1, -1):
+        GR[j] = O.transfer_right_mpo(GR[j + 1], Ws[j], ARk[j], ARb[j])
+    return (ALk, ARk, ALb, ARb), GL, GR
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(CHAINS))
+def test_mpo_matches_kron_hamiltonian(name):
+    Ws, H = CHAINS[name][0](), CHAINS[name][1]()
+    assert np.abs(B.mpo_to_dense(Ws) - H.toarray()).max() < 1e-13
+
+
+@pytest.mark.parametrize("name", list(CHAINS))
+def test_generic_applies_vs_projected_dense_H(name):
+    """P_bra^T H P_ket x == derivative apply, with bra != ket (mpo_derivatives.jl)."""
+    rng = np.random.default_rng(11)
+    Ws, H = CHAINS[name][0](), CHAINS[name][1]().toarray()
+    ket, bra = B.random_mps(L, 2, 6, rng), B.random_mps(L, 2, 6, rng)
+    (ALk, ARk, ALb, ARb), GL, GR = envs(Ws, ket, bra)
+    for j in range(L - 1):
+        x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 2].shape[0], 2))
+        y = O.ac2_apply_generic(GL[j], Ws[j], Ws[j + 1], GR[j + 2], x)
+        yb = B.isometry_ac2(ALb, ARb, j, 2).T @ H @ B.isometry_ac2(ALk, ARk, j, 2) @ x.reshape(-1, order="F")
+        assert rel(y.reshape(-1, order="F"), yb) < 1e-13
+        Lm, Rm = O.prepare_ac2_generic(GL[j], Ws[j], Ws[j + 1], GR[j + 2])
+        assert rel(O.apply_prepared_ac2(Lm, Rm, x), y) < 1e-13
+    for j in range(L):
+        x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 1].shape[0]))
+        y = O.ac_apply_generic(GL[j], Ws[j], GR[j + 1], x)
+        yb = B.isometry_ac(ALb, ARb, j, 2).T @ H @ B.isometry_ac(ALk, ARk, j, 2) @ x.reshape(-1, order="F")
+        assert rel(y.reshape(-1, order="F"), yb) < 1e-13
+        Lm, Rm = O.prepare_ac_generic(GL[j], Ws[j], GR[j + 1])
+        assert rel(O.apply_prepared_ac(Lm, Rm, x), y) < 1e-13
+    for j in range(1, L):
+        x = rng.standard_normal((GL[j].shape[2], GR[j].shape[0]))
+        y = O.c_apply(GL[j], GR[j], x)
+        yb = B.isometry_c(ALb, ARb, j).T @ H @ B.isometry_c(ALk, ARk, j) @ x.reshape(-1, order="F")
+        assert rel(y.reshape(-1, order="F"), yb) < 1e-13
+        Lm, Rm = O.prepare_c(GL[j], GR[j])
+        assert rel(O.apply_prepared_c(Lm, Rm, x), y) < 1e-13
+
+
+@pytest.mark.parametrize("name", list(CHAINS))
+def test_jordan_applies_vs_projected_dense_H(name):
+    """JordanMPO_AC2/AC (hamiltonian_derivatives.jl) with bra == ket, every bond."""
+    rng = np.random.default_rng(12)
+    Ws, H = CHAINS[name][0](), CHAINS[name][1]().toarray()
+    psi = B.random_mps(L, 2, 6, rng)
+    (AL, AR, _, _), GL, GR = envs(Ws, psi, psi)
+    for j in range(L - 1):
+        x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 2].shape[0], 2))
+        P = B.isometry_ac2(AL, AR, j, 2)
+        T = O.jordan_ac2_prepare(GL[j], Ws[j], Ws[j + 1], GR[j + 2])
+        assert rel(O.jordan_ac2_apply(T, x).reshape(-1, order="F"), P.T @ H @ P @ x.reshape(-1, order="F")) < 1e-13
+    for j in range(L):
+        x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 1].shape[0]))
+        P = B.isometry_ac(AL, AR, j, 2)
+        T = O.jordan_ac_prepare(GL[j], Ws[j], GR[j + 1])
+        assert rel(O.jordan_ac_apply(T, x).reshape(-1, order="F"), P.T @ H @ P @ x.reshape(-1, order="F")) < 1e-13
+
+
+def test_jordan_uses_identity_corner_levels():
+    """Jordan applies ignore GL's "not started" and GR's "finished" levels (taken
+    as identities), so they equal the generic apply with those levels set to 1."""
+    rng = np.random.default_rng(3)
+    W = M.xxz_bulk(0.8)
+    D = 6
+    GL, GR = rng.standard_normal((D, 5, D)), rng.standard_normal((D, 5, D))
+    x = rng.standard_normal((D, 2, D, 2))
+    GLi, GRi = GL.copy(), GR.copy()
+    GLi[:, 0, :] = np.eye(D)
+    GRi[:, -1, :] = np.eye(D)
+    yj = O.jordan_ac2_apply(O.jordan_ac2_prepare(GL, W, W, GR), x)
+    assert rel(yj, O.ac2_apply_generic(GLi, W, W, GRi, x)) < 1e-13
+    xa = rng.standard_normal((D, 2, D))
+    ya = O.jordan_ac_apply(O.jordan_ac_prepare(GL, W, GR), xa)
+    assert rel(ya, O.ac_apply_generic(GLi, W, GRi, xa)) < 1e-13
+
+
+def test_jordan_folding_covers_all_terms_j1j2():
+    """J1-J2 has an A block: CA/AB folding paths (hamiltonian_derivatives.jl:379-462)."""
+    rng = np.random.default_rng(4)
+    W = M.j1j2_bulk(1.0, 0.4)
+    D = 5
+    GL, GR = rng.standard_normal((D, 8, D)), rng.standard_normal((D, 8, D))
+    GL[:, 0, :] = np.eye(D)
+    GR[:, -1, :] = np.eye(D)
+    T = O.jordan_ac2_prepare(GL, W, W, GR)
+    assert T["CA"] is not None and T["AB"] is not None and T["IC"] is None and T["BE"] is None
+    x = rng.standard_normal((D, 2, D, 2))
+    assert rel(O.jordan_ac2_apply(T, x), O.ac2_apply_generic(GL, W, W, GR, x)) < 1e-13
+
+
+def test_planted_transposition_is_caught():
+    """Swapping the bra/ket legs of GL (a plausible layout slip) fails the pin."""
+    rng = np.random.default_rng(5)
+    Ws, H = M.xxz(L, 0.6), B.dense_xxz(L, 0.6).toarray()
+    ket, bra = B.random_mps(L, 2, 6, rng), B.random_mps(L, 2, 6, rng)
+    (ALk, ARk, ALb, ARb), GL, GR = envs(Ws, ket, bra)
+    j = 3
+    x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 2].shape[0], 2))
+    yb = B.isometry_ac2(ALb, ARb, j, 2).T @ H @ B.isometry_ac2(ALk, ARk, j, 2) @ x.reshape(-1, order="F")
+    bad = O.ac2_apply_generic(np.transpose(GL[j], (2, 1, 0)), Ws[j], Ws[j + 1], GR[j + 2], x)
+    assert rel(bad.reshape(-1, order="F"), yb) > 1e-2
+
+
+def test_transfer_fixed_point_and_regularize():
+    """l_LL * T_LL = l_LL for a left-canonical state (test/states/infinitemps.jl:38-46)
+    and regularize! removes the projection (transfermatrix.jl:70-80)."""
+    rng = np.random.default_rng(6)
+    D = 5
+    Q, _ = np.linalg.qr(rng.standard_normal((D * 2, D)))
+    A = Q.reshape(D, 2, D, order="F")                 # left isometry
+    I = np.eye(D)
+    assert rel(O.transfer_left_mps(I, A, A), I) < 1e-13
+    v = rng.standard_normal((D, D))
+    lvec, rvec = rng.standard_normal((D, D)), rng.standard_normal((D, D))
+    lvec /= np.einsum("ij,ji->", lvec, rvec)
+    r = O.regularize(v, lvec, rvec)
+    assert abs(np.einsum("ij,ji->", lvec, r)) < 1e-12
+    v3 = rng.standard_normal((D, 3, D))
+    r3 = O.regularize(v3, lvec, rvec)
+    assert np.abs(np.einsum("ij,jgi->g", lvec, r3)).max() < 1e-12
+
+
+def test_tfim_closed_form_energy():
+    for Lc in (1, 2, 5, 9):
+        for g in (0.0, 0.5, 1.0, 1.7):
+            e = np.linalg.eigvalsh(B.dense_tfim(Lc, g).toarray())[0]
+            assert abs(e - B.tfim_open_energy(Lc, g)) < 1e-10
+
+
+def test_majumdar_ghosh_dimer_energy():
+    """J2/J1 = 1/2 (Majumdar-Ghosh) open chain of even length L: the nearest-neighbour
+    dimer product state is a ground state with energy -3/4 per singlet, -3L/8."""
+    e = np.linalg.eigvalsh(B.dense_j1j2(8, 1.0, 0.5).toarray())[0]
+    assert abs(e - (-0.375 * 8)) < 1e-10
+
+
+def test_fixtures_reproduced_by_oracle():
+    data = np.load(__import__("pathlib").Path(__file__).with_name("golden") / "apply_vectors.npz")
+    cases = sorted({k.split("/")[0] for k in data.files})
+    assert len(cases) > 30
+    for c in cases:
+        g = {k.split("/")[1]: data[k] for k in data.files if k.startswith(c + "/")}
+        kind = c.split("_")[0]
+        if kind == "ac2gen":
+            y = O.ac2_apply_generic(g["GL"], g["W1"], g["W2"], g["GR"], g["x"])
+        elif kind == "acgen":
+            y = O.ac_apply_generic(g["GL"], g["W"], g["GR"], g["x"])
+        elif kind == "c":
+            y = O.c_apply(g["GL"], g["GR"], g["x"])
+        elif kind in ("ac2jor", "ac2chain"):
+            y = O.jordan_ac2_apply(O.jordan_ac2_prepare(g["GL"], g["W1"], g["W2"], g["GR"]), g["x"])
+        elif kind in ("acjor", "acchain"):
+            y = O.jordan_ac_apply(O.jordan_ac_prepare(g["GL"], g["W"], g["GR"]), g["x"])
+        elif kind == "tlmps":
+            y = O.transfer_left_mps(g["v"], g["A"], g["Ab"])
+        elif kind == "trmps":
+            y = O.transfer_right_mps(g["v"], g["A"], g["Ab"])
+        elif kind == "tlmpo":
+            y = O.transfer_left_mpo(g["x"], g["O"], g["A"], g["Ab"])
+        elif kind == "trmpo":
+            y = O.transfer_right_mpo(g["v"], g["O"], g["A"], g["Ab"])
+        else:
+            raise AssertionError(c)
+        assert rel(y, g["y"]) < 1e-13, c
