@@ -193,6 +193,16 @@ mk_status mk_eigsolve(mk_op op, double* x, int krylovdim, int maxiter, double to
                       int precision, double* eigval, double* resnorm, int* numops,
                       int* converged);
 
+/* ---- plain GEMM ---------------------------------------------------------------
+ * No single counterpart in the reference (BLAS gemm under TensorOperations'
+ * tensorcontract!, src/utility/utility.jl:295-301, is what every contraction
+ * above reduces to).  Column-major C = alpha op(A) op(B) + beta C through the
+ * library's MFMA kernels in the requested arithmetic, for callers that hold
+ * fused matrices of their own. */
+mk_status mk_gemm(mk_ctx ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
+                  double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                  double beta, double* C, int64_t ldc, int precision);
+
 /* ---- kernel timing ---------------------------------------------------------
  * No counterpart in the reference (its TimerOutputs sections time whole
  * algorithm steps, src/algorithms/groundstate/dmrg.jl:139-150).  When enabled,

@@ -172,7 +172,7 @@ extern "C" mk_status mk_ctx_destroy(mk_ctx h) {
   auto* c = reinterpret_cast<Ctx*>(h);
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  delete c;
+  c->release();
   return MK_OK;
 }
 
@@ -297,4 +297,36 @@ extern "C" mk_status mk_regularize(mk_ctx h, double* v, const double* lvec, cons
   void* lam = nullptr;
   MK_TRY(ctx->small((size_t)w * 8, &lam));
   return k_regularize(v, lvec, rvec, D1, (int)w, D2, static_cast<double*>(lam), ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: plain GEMM through the library's kernels
+// ---------------------------------------------------------------------------
+
+extern "C" mk_status mk_gemm(mk_ctx h, int transa, int transb, int64_t M, int64_t N, int64_t K, double alpha,
+                             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                             int64_t ldc, int precision) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(M >= 0 && N >= 0 && K >= 0 && M < (1ll << 31) && N < (1ll << 31) && K < (1ll << 31), MK_BAD_SHAPE,
+             "gemm: bad dimensions");
+  MK_REQUIRE(A && B && C, MK_BAD_ARG, "gemm: null pointer");
+  MK_REQUIRE(lda >= (transa ? K : M) && ldb >= (transb ? N : K) && ldc >= M, MK_BAD_SHAPE, "gemm: leading dims");
+  MK_REQUIRE(precision >= 0 && precision <= 2, MK_BAD_ARG, "gemm: precision %d", precision);
+  if (M == 0 || N == 0) return MK_OK;
+  ProbSpec p;
+  p.M = M; p.N = N; p.C = C; p.ldc = ldc; p.alpha = alpha; p.beta = beta;
+  if (precision == MK_FP64) {
+    p.add(A, lda, B, ldb, K);
+    return gemm_f64(&p, 1, GemmShape{transa != 0, transb != 0}, ctx->stream);
+  }
+  // K-contiguous copies: stored A is M x K (convert with transposition) or,
+  // transposed, K x M (plain); stored B is K x N (plain) or N x K (transposed)
+  void *ma = nullptr, *mb = nullptr;
+  MK_TRY(carve(ctx, {{view_bytes(transa ? K : M, transa ? M : K, transa ? 'B' : 'A', precision), &ma},
+                     {view_bytes(transb ? N : K, transb ? K : N, transb ? 'A' : 'B', precision), &mb}}));
+  MixedView va, vb;
+  MK_TRY(convert_view(ctx, A, transa ? K : M, transa ? M : K, lda, transa ? 'B' : 'A', precision, &va, ma));
+  MK_TRY(convert_view(ctx, B, transb ? N : K, transb ? K : N, ldb, transb ? 'A' : 'B', precision, &vb, mb));
+  p.add(va.hi, va.kpad, vb.hi, vb.kpad, K, va.lo, vb.lo);
+  return run_gemm(&p, 1, precision, GemmShape{}, ctx->stream);
 }

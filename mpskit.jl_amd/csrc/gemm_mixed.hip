@@ -176,8 +176,11 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            const float a = __builtin_bit_cast(float, fa[0][i][sub]);
-            const float b = __builtin_bit_cast(float, fb[0][j][sub]);
+            // extract the element to a scalar first: bit-casting a vector-element
+            // lvalue reads element 0
+            const unsigned int ua = fa[0][i][sub], ub = fb[0][j][sub];
+            const float a = __uint_as_float(ua);
+            const float b = __uint_as_float(ub);
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
           }
     }

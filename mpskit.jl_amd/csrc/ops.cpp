@@ -336,6 +336,13 @@ using namespace mk;
 
 namespace {
 
+void drop(Op* op) {
+  Ctx* ctx = op->ctx;
+  (void)hipStreamSynchronize(ctx->stream);
+  delete op;
+  ctx->release();
+}
+
 mk_status new_op(mk_ctx h, OpKind kind, int precisions, Op** out) {
   MK_REQUIRE(h != nullptr, MK_BAD_ARG, "null context");
   MK_REQUIRE(out != nullptr, MK_BAD_ARG, "null output handle");
@@ -344,6 +351,7 @@ mk_status new_op(mk_ctx h, OpKind kind, int precisions, Op** out) {
   MK_CHECK_HIP(hipSetDevice(ctx->device));
   Op* op = new Op();
   op->ctx = ctx;
+  ctx->retain();
   op->kind = kind;
   op->precisions = precisions | MK_PREC_FP64;  // FP64 is always available (refinement)
   *out = op;
@@ -352,8 +360,12 @@ mk_status new_op(mk_ctx h, OpKind kind, int precisions, Op** out) {
 
 mk_status finish(Op* op, mk_op* out) {
   mk_status s = make_all_mixed(op);
+  if (s == MK_OK && hipStreamSynchronize(op->ctx->stream) != hipSuccess) {
+    set_error("prepare: device error");
+    s = MK_HIP_ERROR;
+  }
   if (s != MK_OK) {
-    delete op;
+    drop(op);
     return s;
   }
   *out = reinterpret_cast<mk_op>(op);
@@ -364,7 +376,7 @@ mk_status finish(Op* op, mk_op* out) {
   do {                          \
     mk_status _s = (expr);      \
     if (_s != MK_OK) {          \
-      delete op;                \
+      drop(op);                 \
       return _s;                \
     }                           \
   } while (0)
@@ -385,7 +397,7 @@ extern "C" mk_status mk_prepare_c(mk_ctx h, const double* GL, const double* GR, 
   OP_TRY(k_swap23(GL, Da, w, Df, g.L.f64(), st));
   OP_TRY(g.R.alloc('B', Di, w * Dc));
   hipError_t e = hipMemcpyAsync(g.R.f64(), GR, (size_t)Di * w * Dc * 8, hipMemcpyDeviceToDevice, st);
-  if (e != hipSuccess) { delete op; set_error("prepare_c: copy failed"); return MK_HIP_ERROR; }
+  if (e != hipSuccess) { drop(op); set_error("prepare_c: copy failed"); return MK_HIP_ERROR; }
   g.Mx = Df; g.Ki = Di; g.nk = w; g.Ny = Dc; g.Mo = Da;
   op->x_elems = Df * Di;
   op->y_elems = Da * Dc;
@@ -416,8 +428,8 @@ extern "C" mk_status mk_prepare_ac_generic(mk_ctx h, const double* GL, const dou
                   (int)wr, -1, g.L.f64(), st));
   OP_TRY(g.R.alloc('B', Di, wr * Dc));
   hipError_t e = hipMemcpyAsync(g.R.f64(), GR, (size_t)Di * wr * Dc * 8, hipMemcpyDeviceToDevice, st);
-  if (e != hipSuccess) { delete op; set_error("prepare_ac: copy failed"); return MK_HIP_ERROR; }
-  if (hipStreamSynchronize(st) != hipSuccess) { delete op; set_error("prepare_ac: sync failed"); return MK_HIP_ERROR; }
+  if (e != hipSuccess) { drop(op); set_error("prepare_ac: copy failed"); return MK_HIP_ERROR; }
+  if (hipStreamSynchronize(st) != hipSuccess) { drop(op); set_error("prepare_ac: sync failed"); return MK_HIP_ERROR; }
   g.Mx = Df * d; g.Ki = Di; g.nk = wr; g.Ny = Dc; g.Mo = Da * d;
   op->x_elems = Df * d * Di;
   op->y_elems = Da * d * Dc;
@@ -670,8 +682,10 @@ extern "C" mk_status mk_op_info(mk_op h, int64_t* x_elems, int64_t* y_elems, dou
 extern "C" mk_status mk_op_free(mk_op h) {
   if (!h) return MK_OK;
   Op* op = reinterpret_cast<Op*>(h);
-  (void)hipSetDevice(op->ctx->device);
-  (void)hipStreamSynchronize(op->ctx->stream);
+  Ctx* ctx = op->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete op;
+  ctx->release();
   return MK_OK;
 }

@@ -1,5 +1,6 @@
 // Internal structures of contexts and prepared derivative operators.
 #pragma once
+#include <atomic>
 #include <initializer_list>
 #include <utility>
 #include <vector>
@@ -24,6 +25,13 @@ struct DevBuf {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // operators keep their context alive: mk_ctx_destroy drops the caller's
+  // reference, the context is deleted when the last operator is freed
+  std::atomic<int> refs{1};
+  void retain() { refs.fetch_add(1); }
+  void release() {
+    if (refs.fetch_sub(1) == 1) delete this;
+  }
   DevBuf arena;      // stream-ordered scratch for applies / transfers
   DevBuf smallbuf;   // small stream-ordered scratch (reduction partials, scalars)
   mk_status scratch(size_t nbytes, void** out);

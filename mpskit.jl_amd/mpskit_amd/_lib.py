@@ -65,6 +65,7 @@ _SIGS = {
     "mk_regularize": ([_P, _P, _P, _P, _I64, _I64, _I64], _INT),
     "mk_eigsolve": ([_P, _P, _INT, _INT, _D, _INT, C.POINTER(_D), C.POINTER(_D), C.POINTER(_INT),
                      C.POINTER(_INT)], _INT),
+    "mk_gemm": ([_P, _INT, _INT, _I64, _I64, _I64, _D, _P, _I64, _P, _I64, _D, _P, _I64, _INT], _INT),
     "mk_prof_enable": ([_INT], _INT),
     "mk_prof_reset": ([], _INT),
     "mk_prof_query": ([C.c_char_p, C.POINTER(_D), C.POINTER(_I64)], _INT),

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import threading
 
 import torch
@@ -39,6 +40,8 @@ class Context:
 
     def __del__(self):
         h = getattr(self, "handle", None)
+        if sys.is_finalizing():      # the HIP runtime may already be torn down
+            return
         if h is not None and h.value:
             try:
                 self._lib.mk_ctx_destroy(h)

@@ -21,6 +21,7 @@ MFMA with FP32 accumulation); inputs and outputs are float64 in every mode.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from typing import Iterable
 
 import torch
@@ -107,6 +108,8 @@ class PreparedDerivative(DerivativeOperator):
 
     def free(self) -> None:
         h = getattr(self, "handle", None)
+        if sys.is_finalizing():      # the HIP runtime may already be torn down
+            return
         if h is not None and h.value:
             load().mk_op_free(h)
             self.handle = None
@@ -237,3 +240,20 @@ def prepare_operator(H, precisions=None) -> PreparedDerivative:
                                         wr, Dr, mask, C.byref(h)))
         return PreparedDerivative(h, ctx, (Dl, d1, Dr, d2), (Dl, d1, Dr, d2), "JordanAC2", mask)
     raise TypeError(f"cannot prepare {type(H).__name__}")
+
+
+def gemm(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor | None = None, alpha: float = 1.0, beta: float = 0.0,
+         transa: bool = False, transb: bool = False, precision: PrecisionArg = "fp64") -> torch.Tensor:
+    """Column-major C = alpha op(A) op(B) + beta C through the library's MFMA kernels."""
+    A, B = T.as_colmajor(A), T.as_colmajor(B)
+    M, K = (A.shape[1], A.shape[0]) if transa else (A.shape[0], A.shape[1])
+    K2, N = (B.shape[1], B.shape[0]) if transb else (B.shape[0], B.shape[1])
+    if K != K2:
+        raise SpaceMismatch(1, f"gemm: inner dimensions {K} != {K2}")
+    if C is None:
+        C = T.empty((M, N), A.device)
+        beta = 0.0
+    ctx = default_context()
+    check(load().mk_gemm(ctx.handle, int(transa), int(transb), M, N, K, float(alpha), T.ptr(A), max(A.shape[0], 1),
+                         T.ptr(B), max(B.shape[0], 1), float(beta), T.ptr(C), max(C.shape[0], 1), _prec(precision)))
+    return C

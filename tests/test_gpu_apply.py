@@ -72,6 +72,7 @@ def build_op(mk, kind, g):
 
 def test_fixture_vectors_all_precisions(mk, fixtures):
     worst = {p: 0.0 for p in PRECS}
+    failures = []
     for case, g in sorted(fixtures.items()):
         kind = case.split("_")[0]
         print("case", case, flush=True)
@@ -86,14 +87,17 @@ def test_fixture_vectors_all_precisions(mk, fixtures):
             elif kind == "trmpo":
                 y = mk.transfer_right(d["v"], d["O"], d["A"], d["Ab"])
             e = rel(host(mk, y), g["y"])
-            assert e < TOL["fp64"], (case, e)
+            if not e < TOL["fp64"]:
+                failures.append((case, "fp64", e))
             worst["fp64"] = max(worst["fp64"], e)
             continue
         for p in PRECS:
             e = rel(host(mk, op(d["x"], precision=p)), g["y"])
             worst[p] = max(worst[p], e)
-            assert e < TOL[p], (case, p, e)
+            if not e < TOL[p]:
+                failures.append((case, p, e))
     print("worst relative errors", worst)
+    assert not failures, failures
 
 
 @pytest.mark.parametrize("D", [37, 130, 300, 1030])
