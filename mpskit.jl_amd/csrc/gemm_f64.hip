@@ -200,7 +200,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
     k0 = nk0;
   }
 
-  // epilogue: D lane layout -- column li, rows 4*lk + r
+  // epilogue: v_mfma_f64_16x16x4f64 D lane layout -- lane (li, lk) item r holds
+  // row lk + 4*r, column li (unlike the f32/bf16 16x16 MFMAs, rows 4*lk + r)
   const double alpha = P.alpha, beta = P.beta;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
       if (col >= P.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + lk * 4 + r;
+        const int row = m0 + wm0 + i * 16 + lk + 4 * r;
         if (row >= P.M) continue;
         double v = acc[i][j][r];
         if (P.epi == EPI_CB) {

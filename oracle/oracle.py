@@ -69,19 +69,25 @@ def _es(spec, *ops):
 
 def c_apply(GL, GR, x):
     """Zero-site derivative y[a,c] = GL[a,g,f] x[f,i] GR[i,g,c] (mpo_derivatives.jl:92-97)."""
-    return _es("agf,fi,igc->ac", GL, x, GR)
+    return _es("agi,igc->ac", _es("agf,fi->agi", GL, x), GR)
 
 
 def ac_apply_generic(GL, W, GR, x):
     """Single-site derivative (mpo_derivatives.jl:99-106):
     y[-1 -2; -3] = GL[-1 5; 4] x[4 2; 1] W[5 -2; 2 3] GR[1 3; -3]."""
-    return _es("agf,fhi,gbhl,ilc->abc", GL, x, W, GR)
+    t = _es("agf,fhi->aghi", GL, x)
+    t = _es("aghi,gbhl->abil", t, W)
+    return _es("abil,ilc->abc", t, GR)
 
 
 def ac2_apply_generic(GL, W1, W2, GR, x):
     """Two-site derivative (mpo_derivatives.jl:144-152):
     y[-1 -2; -3 -4] = GL[-1 7; 6] x[6 5; 1 3] W1[7 -2; 5 4] W2[4 -4; 3 2] GR[1 2; -3]."""
-    return _es("agf,fhij,gbhk,kejl,ilc->abce", GL, x, W1, W2, GR)
+    # pairwise, in the order L (x R) of the prepared form
+    t = _es("fhij,ilc->fhjlc", x, GR)
+    t = _es("fhjlc,kejl->fhkce", t, W2)
+    t = _es("fhkce,gbhk->fgbce", t, W1)
+    return _es("fgbce,agf->abce", t, GL)
 
 
 # ---------------------------------------------------------------------------
@@ -316,24 +322,28 @@ def jordan_ac_apply(T, x):
 def transfer_left_mps(v, A, Ab):
     """transfer_left(v, A, Abar) for a bond tensor v (transfer.jl:18-31):
     y[a,c] = v[m,n] A[n,h,c] conj(Ab[m,h,a])."""
-    return _es("mn,nhc,mha->ac", v, A, np.conj(Ab))
+    return _es("mhc,mha->ac", _es("mn,nhc->mhc", v, A), np.conj(Ab))
 
 
 def transfer_right_mps(v, A, Ab):
     """transfer_right(v, A, Abar) (transfer.jl:43-55): y[a,b] = A[a,h,i] conj(Ab[b,h,j]) v[i,j]."""
-    return _es("ahi,bhj,ij->ab", A, np.conj(Ab), v)
+    return _es("ahj,bhj->ab", _es("ahi,ij->ahj", A, v), np.conj(Ab))
 
 
 def transfer_left_mpo(x, O, A, Ab):
     """MPO transfer_left (transfer.jl:125-127):
     y[-1 -2; -3] = x[1 2; 4] A[4 5; -3] O[2 3; 5 -2] conj(Ab[1 3; -1])."""
-    return _es("mgn,nhc,gkhb,mka->abc", x, A, O, np.conj(Ab))
+    t = _es("mgn,nhc->mghc", x, A)
+    t = _es("mghc,gkhb->mkbc", t, O)
+    return _es("mkbc,mka->abc", t, np.conj(Ab))
 
 
 def transfer_right_mpo(v, O, A, Ab):
     """MPO transfer_right (transfer.jl:128-130):
     y[-1 -2; -3] = A[-1 2; 1] O[-2 4; 2 3] conj(Ab[-3 4; 5]) v[1 3; 5]."""
-    return _es("ahi,bkhl,cko,ilo->abc", A, O, np.conj(Ab), v)
+    t = _es("ahi,ilo->ahlo", A, v)
+    t = _es("ahlo,bkhl->abko", t, O)
+    return _es("abko,cko->abc", t, np.conj(Ab))
 
 
 def regularize(v, lvec, rvec):

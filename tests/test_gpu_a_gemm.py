@@ -54,3 +54,19 @@ def test_gemm_beta_zero_ignores_nan_output(env):
         mk.gemm(A, B, C, beta=0.0, precision=prec)
         torch.cuda.synchronize()
         assert bool(torch.isfinite(C).all()), prec
+
+
+def test_mfma_output_lane_layout(env):
+    """One 16x16x4 block with rank-1 structured operands: C[i,j] = (i+1) * 100 (j+1)
+    pins the MFMA output row/column assignment of every precision."""
+    mk, torch = env
+    T = mk.tensors
+    A = torch.zeros(16, 4, dtype=torch.float64, device="cuda")
+    A[:, 0] = torch.arange(16, dtype=torch.float64, device="cuda") + 1
+    B = torch.zeros(4, 16, dtype=torch.float64, device="cuda")
+    B[0, :] = 100 * (torch.arange(16, dtype=torch.float64, device="cuda") + 1)
+    ref = A @ B
+    for prec in ("fp64", "fp32", "bf16x3"):
+        C = mk.gemm(T.as_colmajor(A), T.as_colmajor(B), precision=prec)
+        torch.cuda.synchronize()
+        assert float((C - ref).abs().max()) == 0.0, prec
