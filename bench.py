@@ -31,6 +31,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT), str(ROOT / "mpskit.jl_amd")]
 
 PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2516.6 / 3.0}
+KERNEL = {"fp64": "gemm_f64", "fp32": "gemm_f32", "bf16x3": "gemm_bf16x3"}
 
 
 def jordan_w(rng, d=2, nmid=3):
@@ -132,7 +133,7 @@ def main():
         ms = time_applies(op, x, y, prec, args.steps, args.warmup)
         lib.mk_prof_enable(0)
         tot, cnt = ctypes.c_double(), ctypes.c_int64()
-        lib.mk_prof_query(f"gemm_{prec}".encode(), ctypes.byref(tot), ctypes.byref(cnt))
+        lib.mk_prof_query(KERNEL[prec].encode(), ctypes.byref(tot), ctypes.byref(cnt))
         gemm_ms = tot.value / max(cnt.value, 1)
         if prec == "fp64":
             y64 = y.clone()
