@@ -23,3 +23,9 @@ __all__ = [
     "PreparedDerivative", "prepare_operator", "ProductTransferMatrix", "RegTransferMatrix",
     "SingleTransferMatrix", "TransferMatrix", "regularize", "regularize_", "transfer_left", "transfer_right",
 ]
+from .algorithms import DMRG2, AdaptiveKrylov, Lanczos, expectation_value, find_groundstate
+from .environments import FiniteEnvironments, environments
+from .states import FiniteMPOHamiltonian, FiniteMPS
+
+__all__ += ["DMRG2", "AdaptiveKrylov", "Lanczos", "expectation_value", "find_groundstate", "FiniteEnvironments",
+            "environments", "FiniteMPOHamiltonian", "FiniteMPS"]

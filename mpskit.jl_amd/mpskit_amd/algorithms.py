@@ -1,0 +1,186 @@
+"""DMRG2 driver on the HIP path (src/algorithms/groundstate/dmrg.jl).
+
+Reproduces the reference's control flow: per bond, build the Jordan two-site
+derivative from lazily updated environments, measure the local Galerkin error,
+solve the local eigenproblem with an adaptive Krylov budget
+(``AdaptiveKrylov``, src/utility/dynamictols.jl:204-235), split the result by a
+truncated SVD (``gauge2!``, src/states/orthoview.jl:401-420) and sweep
+left-right-left until the sweep-wide error reaches the tolerance or the
+truncation error (dmrg.jl:286-354).  Contractions, environment updates and the
+Krylov solver are the HIP library's; QR/SVD of the gauge steps are float64 on
+the GPU (rocSOLVER via torch.linalg); energies are float64.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import torch
+
+from . import tensors as T
+from .derivatives import JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, prepare_operator
+from .environments import FiniteEnvironments
+from .states import FiniteMPS, qr_left
+
+
+@dataclass
+class AdaptiveKrylov:
+    """Per-bond Krylov budget controller (dynamictols.jl:182-235)."""
+    krylovdim_min: int = 3
+    krylovdim_max: int = 20
+    iter_min: int = 1
+    iter_max: int = 1
+    tol_min: float = 1e-12
+    tol_max: float = 1e-2
+    truncation_factor: float = 0.1
+    tol_factor: float = 0.1
+
+    def adapt_solver(self, decay_rate=0.0, g_local=0.0, g_global=0.0, eps_trunc=0.0):
+        """Returns (tol, krylovdim, maxiter) as ``adapt_solver`` (dynamictols.jl:204-235)."""
+        tol = min(max(max(self.truncation_factor * eps_trunc, self.tol_factor * g_global), self.tol_min),
+                  self.tol_max)
+        rho = min(max(decay_rate, 0.0), 1.0)
+        if not (0.0 < rho < 1.0):
+            if rho == 0.0:
+                return tol, self.krylovdim_min, self.iter_min
+            return tol, self.krylovdim_max, self.iter_max
+        R = max(g_local, self.tol_min) / tol
+        nmatvecs = int(round(math.log(R) / math.log(1.0 / rho))) if R > 0 else 0
+        krylovdim = min(max(nmatvecs, self.krylovdim_min), self.krylovdim_max)
+        maxiter = min(max(-(-5 * nmatvecs // (2 * krylovdim)) - 3, self.iter_min), self.iter_max)
+        return tol, krylovdim, maxiter
+
+
+@dataclass
+class Lanczos:
+    """Fixed Krylov settings (KrylovKit.Lanczos as used by fixedpoint, fixedpoint.jl:15-18)."""
+    tol: float = 1e-10
+    krylovdim: int = 30
+    maxiter: int = 100
+
+    def adapt_solver(self, **kw):
+        return self.tol, self.krylovdim, self.maxiter
+
+
+@dataclass
+class DMRG2:
+    """Two-site DMRG (dmrg.jl:171-209).  ``trunc_dim`` is the maximal bond
+    dimension kept by the truncated SVD (``truncrank``) and ``trunc_err`` an
+    optional cut on discarded weight; ``precision`` selects the arithmetic of
+    the Krylov applies ("fp64", "fp32", "bf16x3"; residuals stay float64)."""
+    tol: float = 1e-10
+    maxiter: int = 100
+    verbosity: int = 1
+    alg_eigsolve: object = field(default_factory=AdaptiveKrylov)
+    trunc_dim: int = 64
+    trunc_err: float = 0.0
+    precision: str = "fp64"
+    exact_sweeps: bool = False   # run exactly `maxiter` sweeps (no early stop)
+
+
+def project_complement(Y, X):
+    """Y - X (X^T Y) for a left isometry X (utility.jl:22)."""
+    Xm = T.fuse(X, 2)
+    Ym = T.fuse(Y, 2)
+    return Ym - Xm @ (Xm.t() @ Ym)
+
+
+def gauge2(psi: FiniteMPS, pos, direction, ac2, trunc_dim, trunc_err):
+    """Truncated SVD split of the two-site center (``gauge2!``, orthoview.jl:401-420)
+    with ``normalize = true``; returns the discarded-weight 2-norm."""
+    Dl, d1, Dr, d2 = ac2.shape
+    M = T.fuse(ac2, 2)
+    U, S, Vh = torch.linalg.svd(M, full_matrices=False)
+    k = min(trunc_dim, S.shape[0])
+    if trunc_err > 0.0:
+        tail = torch.sqrt(torch.flip(torch.cumsum(torch.flip(S * S, [0]), 0), [0]))
+        keep = int((tail > trunc_err).sum().item())
+        k = max(1, min(k, keep))
+    eps = float(torch.linalg.vector_norm(S[k:])) if k < S.shape[0] else 0.0
+    U, S, Vh = U[:, :k], S[:k], Vh[:k]
+    S = S / torch.linalg.vector_norm(S)
+    Al = T.as_colmajor(T.unfuse(U, (Dl, d1, k)))
+    # rows of Vh run over (r, p2) with r fastest -> AR[s, p2, r]
+    ar = T.as_colmajor(T.unfuse(Vh, (k, Dr, d2)).permute(0, 2, 1))
+    if direction == "right":
+        psi.AL[pos] = Al
+        psi.AC = T.as_colmajor(S[:, None, None] * ar)
+        psi.AR[pos] = None
+        psi.AR[pos + 1] = None
+        psi.center = pos + 1
+    else:
+        psi.AR[pos + 1] = ar
+        psi.AC = T.as_colmajor(Al * S[None, None, :])
+        psi.AL[pos] = None
+        psi.AL[pos + 1] = None
+        psi.center = pos
+    return eps
+
+
+def expectation_value(psi: FiniteMPS, H, envs: FiniteEnvironments):
+    """<psi|H|psi> = <AC| dAC |AC> at the center (expval.jl:96-104, mpohamiltonian.jl:922-936)."""
+    c = psi.center
+    op = prepare_operator(JordanMPO_AC_Hamiltonian(envs.leftenv(c, psi), H[c], envs.rightenv(c, psi)))
+    y = op(psi.AC)
+    return float(torch.dot(psi.AC.reshape(-1), y.reshape(-1)) / torch.dot(psi.AC.reshape(-1), psi.AC.reshape(-1)))
+
+
+def local_update(pos, direction, psi, H, alg: DMRG2, envs, eps_global, eps_trunc, decay_rate, stats):
+    """``local_update!(::DMRG2)`` (dmrg.jl:211-244)."""
+    psi.move_center(pos if direction == "right" else pos + 1)
+    GL = envs.leftenv(pos, psi)
+    GR = envs.rightenv(pos + 1, psi)
+    precs = ("fp64",) if alg.precision == "fp64" else ("fp64", alg.precision)
+    Heff = prepare_operator(JordanMPO_AC2_Hamiltonian(GL, H[pos], H[pos + 1], GR), precisions=precs)
+    ac2 = psi.AC2(pos)
+    y = Heff(ac2)
+    y = y / torch.linalg.vector_norm(y)
+    ALp = psi.AL[pos] if psi.center == pos + 1 else qr_left(psi.AC)[0]
+    eps_local = float(torch.linalg.vector_norm(project_complement(y, ALp)))
+    tol, krylovdim, maxiter = alg.alg_eigsolve.adapt_solver(decay_rate=decay_rate, g_local=eps_local,
+                                                            g_global=eps_global, eps_trunc=eps_trunc)
+    krylovdim = max(2, min(krylovdim, ac2.numel()))
+    _, newac2, info = Heff.eigsolve(ac2, krylovdim=krylovdim, maxiter=maxiter, tol=tol, precision=alg.precision)
+    stats["numops"] += info["numops"] + 1
+    eps_trunc = gauge2(psi, pos, direction, newac2, alg.trunc_dim, alg.trunc_err)
+    numops = max(1, info["numops"])
+    decay = (info["normres"] / eps_local) ** (1.0 / numops) if eps_local > 0 else 0.0
+    decay_rate = min(max(decay, 1e-3), 0.999)
+    Heff.free()
+    return eps_local, eps_trunc, decay_rate
+
+
+def find_groundstate(psi: FiniteMPS, H, alg: DMRG2, envs: FiniteEnvironments | None = None):
+    """``find_groundstate(psi, H, DMRG2(...))`` (dmrg.jl:278-356): returns (psi, envs, eps, history)."""
+    L = psi.L
+    envs = envs or FiniteEnvironments(psi, H)
+    n = L - 1
+    eps_locals = [1.0] * n
+    eps_truncs = [0.0] * n
+    decay_rates = [0.0] * n
+    eps_global = max(eps_locals)
+    history = []
+    for it in range(1, alg.maxiter + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = {"numops": 0}
+        for pos in list(range(0, L - 1)):
+            eps_locals[pos], eps_truncs[pos], decay_rates[pos] = local_update(
+                pos, "right", psi, H, alg, envs, eps_global, eps_truncs[pos], decay_rates[pos], stats)
+            eps_global = max(eps_locals)
+        for pos in list(range(L - 3, -1, -1)):
+            eps_locals[pos], eps_truncs[pos], decay_rates[pos] = local_update(
+                pos, "left", psi, H, alg, envs, eps_global, eps_truncs[pos], decay_rates[pos], stats)
+            eps_global = max(eps_locals)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        E = expectation_value(psi, H, envs)
+        history.append(dict(iter=it, energy=E, eps=eps_global, trunc=max(eps_truncs), time=dt,
+                            numops=stats["numops"]))
+        if alg.verbosity >= 2:
+            print(f"DMRG2 {it:3d}: E = {E:.15f}  err = {eps_global:.3e}  trunc = {max(eps_truncs):.2e}  "
+                  f"time = {dt:.3f}s", flush=True)
+        if not alg.exact_sweeps and eps_global <= max(alg.tol, max(eps_truncs)):
+            break
+    return psi, envs, eps_global, history

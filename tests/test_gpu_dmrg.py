@@ -1,0 +1,76 @@
+"""DMRG2 on the HIP path: ground-state energies against closed forms and exact
+diagonalisation; mixed-precision sweeps against the float64 sweep.
+
+Tolerances: energy per site 1e-10 where the MPS is exact or
+converged; the L=20 Heisenberg energy of the reference's own test
+(test/algorithms/groundstate.jl:172-174) at its atol 1e-4.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import brute as B  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def mk():
+    import torch
+    assert torch.cuda.is_available()
+    import mpskit_amd
+    mpskit_amd.load()
+    return mpskit_amd
+
+
+def run(mk, Ws, L, D, precision="fp64", sweeps=10, seed=0, exact=False, tol=1e-12):
+    H = mk.FiniteMPOHamiltonian(Ws)
+    psi = mk.FiniteMPS.random(L, 2, D, seed=seed)
+    alg = mk.DMRG2(tol=tol, maxiter=sweeps, trunc_dim=D, precision=precision, exact_sweeps=exact, verbosity=2)
+    psi, envs, eps, hist = mk.find_groundstate(psi, H, alg)
+    return hist[-1]["energy"], hist
+
+
+def test_tfim_small_exact_bond(mk):
+    L, g = 12, 1.0
+    print("dmrg tfim L=12", flush=True)
+    E, hist = run(mk, mk.models.tfim(L, g), L, 64, sweeps=6)
+    assert abs(E - B.tfim_open_energy(L, g)) / L < 1e-10, (E, B.tfim_open_energy(L, g))
+
+
+def test_xxz_vs_exact_diagonalisation(mk):
+    L, delta = 10, 0.7
+    e_ed = np.linalg.eigvalsh(B.dense_xxz(L, delta).toarray())[0]
+    print("dmrg xxz L=10", flush=True)
+    E, _ = run(mk, mk.models.xxz(L, delta), L, 32, sweeps=8)
+    assert abs(E - e_ed) / L < 1e-10, (E, e_ed)
+
+
+def test_j1j2_vs_exact_diagonalisation(mk):
+    """Exercises the CA / AB (A-block) Jordan terms inside a sweep."""
+    L = 10
+    e_ed = np.linalg.eigvalsh(B.dense_j1j2(L, 1.0, 0.24).toarray())[0]
+    print("dmrg j1j2 L=10", flush=True)
+    E, _ = run(mk, mk.models.j1j2(L, 1.0, 0.24), L, 32, sweeps=8)
+    assert abs(E - e_ed) / L < 1e-10, (E, e_ed)
+
+
+def test_heisenberg_L20_reference_energy(mk):
+    """test/algorithms/groundstate.jl:151-175: E = -8.6824724 for the L=20 S=1/2 chain."""
+    print("dmrg heisenberg L=20", flush=True)
+    E, _ = run(mk, mk.models.xxz(20, 1.0), 20, 40, sweeps=8, tol=1e-10)
+    assert abs(E - B.HEISENBERG_L20_OBC) < 1e-4, E
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_tfim_L32_mixed_matches_fp64(mk, precision):
+    """TFIM L=32, g=1, D=64, 10 sweeps: float64 and mixed sweeps both reach the
+    free-fermion energy; per-site difference <= 1e-10."""
+    L, g = 32, 1.0
+    e_exact = B.tfim_open_energy(L, g)
+    print("dmrg tfim L=32 fp64", flush=True)
+    E64, _ = run(mk, mk.models.tfim(L, g), L, 64, sweeps=10, exact=True)
+    print(f"dmrg tfim L=32 {precision}", flush=True)
+    Em, _ = run(mk, mk.models.tfim(L, g), L, 64, precision=precision, sweeps=10, exact=True)
+    print(f"E64={E64:.15f} Emixed={Em:.15f} exact={e_exact:.15f}", flush=True)
+    assert abs(E64 - e_exact) / L < 1e-10
+    assert abs(Em - E64) / L < 1e-10
