@@ -73,13 +73,15 @@ def xxz_bulk(delta=1.0):
 def j1j2_bulk(j1=1.0, j2=0.24):
     """H = j1 sum S_i.S_{i+1} + j2 sum S_i.S_{i+2}  (levels 1,3,3,1 -> w = 8, d = 2).
 
-    Channels 1-3 carry a nearest-neighbour term, channels 4-6 a next-nearest one
-    that is passed through one site by an identity in the A block."""
+    Channels 1-3 start a next-nearest-neighbour term, which an identity in the A
+    block passes to channels 4-6; channels 4-6 also start the nearest-neighbour
+    term, and both end there.  Couplings only go to higher channels (upper
+    triangular), the level order the environment fixed points rely on."""
     a = [SPLUS / np.sqrt(2.0), SMINUS / np.sqrt(2.0), SZ]
     b = [SMINUS / np.sqrt(2.0), SPLUS / np.sqrt(2.0), SZ]
-    C = [j1 * o for o in a] + [j2 * o for o in a]
-    B = b + [np.zeros((2, 2))] * 3
-    A = [(4 + k, 1 + k, np.eye(2)) for k in range(3)]
+    C = [j2 * o for o in a] + [j1 * o for o in a]
+    B = [np.zeros((2, 2))] * 3 + b
+    A = [(1 + k, 4 + k, np.eye(2)) for k in range(3)]
     return _jordan(2, C, B, np.zeros((2, 2)), A)
 
 

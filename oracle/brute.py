@@ -184,3 +184,11 @@ TFIM_INFINITE_G1 = -4.0 / np.pi
 HEISENBERG_INFINITE = 0.25 - np.log(2.0)
 MAJUMDAR_GHOSH = -0.375
 HEISENBERG_L20_OBC = -8.6824724   # test/algorithms/groundstate.jl:172-174 (atol 1e-4)
+
+
+def tfim_infinite_energy(g):
+    """Infinite TFIM (H = -sum s^z s^z - g sum s^x) ground energy per site,
+    e0 = -(1/pi) int_0^pi sqrt(1 + g^2 - 2 g cos k) dk  (Pfeuty)."""
+    from scipy.integrate import quad
+    val, _ = quad(lambda k: np.sqrt(1.0 + g * g - 2.0 * g * np.cos(k)), 0.0, np.pi, epsabs=1e-14, epsrel=1e-14)
+    return -val / np.pi

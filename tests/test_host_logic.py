@@ -32,6 +32,10 @@ def test_finite_chain_boundaries():
     assert all(W.shape == (5, 2, 2, 5) for W in Ws[1:-1])
     Wj = M.j1j2_bulk(1.0, 0.24)
     assert Wj.shape == (8, 2, 2, 8)
-    # pass-through identities of the next-nearest-neighbour channels in the A block
+    # pass-through identities of the next-nearest-neighbour channels in the A block,
+    # upper triangular in channel order
     for k in range(3):
-        assert np.array_equal(Wj[4 + k, :, :, 1 + k], np.eye(2))
+        assert np.array_equal(Wj[1 + k, :, :, 4 + k], np.eye(2))
+    for g in range(8):
+        for k in range(g):
+            assert not np.any(Wj[g, :, :, k]), (g, k)

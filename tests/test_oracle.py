@@ -199,3 +199,12 @@ def test_fixtures_reproduced_by_oracle():
         else:
             raise AssertionError(c)
         assert rel(y, g["y"]) < 1e-13, c
+
+
+def test_tfim_infinite_closed_form_vs_open_chains():
+    """e0(g) from the dispersion integral equals the L -> infinity limit of the open
+    chain closed form (boundary energy cancels in E(2L) - E(L)); g = 1 gives -4/pi."""
+    assert abs(B.tfim_infinite_energy(1.0) - B.TFIM_INFINITE_G1) < 1e-12
+    for g in (0.3, 0.5, 1.5):
+        bulk = (B.tfim_open_energy(800, g) - B.tfim_open_energy(400, g)) / 400
+        assert abs(bulk - B.tfim_infinite_energy(g)) < 1e-8, g
