@@ -105,6 +105,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--D", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
+                    help="time one precision only (for profiler runs); prints no JSON line")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 1 GiB device-to-device five times (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
@@ -114,6 +118,15 @@ def main():
     import torch
     import mpskit_amd as mk
     from mpskit_amd import tensors as T
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("# copied 5 x 1 GiB", file=sys.stderr)
+        return
 
     D, d = args.D, 2
     W = T.from_numpy(jordan_w(np.random.default_rng(3)))
@@ -127,7 +140,7 @@ def main():
 
     res = {}
     y64 = None
-    for prec in ("fp64", "fp32", "bf16x3"):
+    for prec in ((args.only,) if args.only else ("fp64", "fp32", "bf16x3")):
         lib.mk_prof_reset()
         lib.mk_prof_enable(1)
         ms = time_applies(op, x, y, prec, args.steps, args.warmup)
@@ -144,6 +157,8 @@ def main():
         print(f"# {prec}: {ms:.3f} ms/apply, gemm {gemm_ms:.3f} ms, {op.flops / ms / 1e9:.1f} TF/s, "
               f"rel err vs fp64 {err:.2e}", file=sys.stderr, flush=True)
 
+    if args.only:
+        return
     F = op.flops
     head = res["bf16x3"]
     ach = F / (head["gemm_ms"] * 1e-3) / 1e12

@@ -13,7 +13,7 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 typedef double dx2 __attribute__((ext_vector_type(2)));
 
 constexpr int NT = 256;
-constexpr int BK = 16;
+constexpr int BK = 32;
 constexpr int PAD = 16;     // LDS row padding (doubles): alternates 128-B bank halves
 constexpr int NXCD = 8;
 constexpr int GROUP_M = 8;  // tile-row grouping for L2 reuse

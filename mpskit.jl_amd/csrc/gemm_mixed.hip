@@ -32,10 +32,26 @@ constexpr int GROUP_M = 8;
 
 enum Mode { M_BF16X3 = 0, M_F32 = 1 };
 
+bool g_force_small = false;  // test hook: keep the 128/64 kernels for every shape
+
 __device__ __forceinline__ int xcd_remap(int t, int total) {
   const int q = total / NXCD, r = total % NXCD;
   const int x = t % NXCD, local = t / NXCD;
   return x * q + min(x, r) + local;
+}
+
+// Two-site physical (CB) term of one output element y[a,b,c,e] (row = a + Dl b,
+// col = c + Dr e): sum_{h,j} CB[b,e,h,j] x[a,h,c,j].  Outlined with scalar
+// arguments so the unrolled epilogues stay small.
+__device__ __noinline__ double cb_term(const double* __restrict__ cb, const double* __restrict__ x, int Dl, int Dr,
+                                       int d1, int d2, int row, int col) {
+  const int a = row % Dl, bb = row / Dl;
+  const int c = col % Dr, e = col / Dr;
+  double t = 0.0;
+  for (int jj = 0; jj < d2; ++jj)
+    for (int h = 0; h < d1; ++h)
+      t += cb[bb + d1 * (e + d2 * (h + d1 * jj))] * x[a + (int64_t)Dl * (h + d1 * ((int64_t)c + (int64_t)Dr * jj))];
+  return t;
 }
 
 // Load a (R rows x 64 B) slab of a K-contiguous operand into registers.
@@ -203,16 +219,7 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
         const int row = m0 + wm0 + i * 16 + lg * 4 + r;
         if (row >= P.M) continue;
         double v = (double)acc[i][j][r];
-        if (P.epi == EPI_CB) {
-          const int a = row % P.Dl, bb = row / P.Dl;
-          const int c = col % P.Dr, e = col / P.Dr;
-          double tsum = 0.0;
-          for (int jj = 0; jj < P.d2; ++jj)
-            for (int h = 0; h < P.d1; ++h)
-              tsum += P.cb[bb + P.d1 * (e + P.d2 * (h + P.d1 * jj))] *
-                      P.cbx[a + (int64_t)P.Dl * (h + P.d1 * ((int64_t)c + (int64_t)P.Dr * jj))];
-          v += tsum;
-        }
+        if (P.epi == EPI_CB) v += cb_term(P.cb, P.cbx, P.Dl, P.Dr, P.d1, P.d2, row, col);
         double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
         v *= alpha;
         if (beta != 0.0) v += beta * *cp;
@@ -222,7 +229,187 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
   }
 }
 
-template <int BM, int BN, int MODE>
+// ---------------------------------------------------------------------------
+// Large-tile variant: 256 x 256 output tile, 512 threads = 8 waves (2 x 4), each
+// wave 128 x 64.  Halves the operand re-reads of the 128 x 128 kernel and
+// doubles the MFMA work per LDS fragment.  LDS rows hold 64 B of K without
+// padding; 16-B chunks are XOR-swizzled by (row >> 2) & 3 so that the 16 rows
+// read by one 16-lane group hit distinct banks.  Double-buffered: 128 KB (bf16).
+// ---------------------------------------------------------------------------
+
+constexpr int BIG = 256;
+constexpr int BNT = 512;
+
+__device__ __forceinline__ int swz(int r, int q) { return r * ROWB + ((q ^ ((r >> 2) & 3)) << 4); }
+
+template <int R>
+__device__ __forceinline__ void load_slab_big(const char* __restrict__ base, int64_t ld_bytes, int r0,
+                                              int64_t kbyte0, int Rlim, int64_t Kbytes,
+                                              ux4 (&reg)[R * ROWB / 16 / BNT]) {
+  constexpr int CH = R * ROWB / 16 / BNT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * BNT;
+    const int r = id >> 2, q = id & 3;
+    const int gr = r0 + r;
+    const int64_t kb = kbyte0 + q * 16;
+    ux4 v = {0u, 0u, 0u, 0u};
+    if (gr < Rlim && kb < Kbytes) v = *reinterpret_cast<const ux4*>(base + (int64_t)gr * ld_bytes + kb);
+    reg[c] = v;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void store_slab_big(char* S, const ux4 (&reg)[R * ROWB / 16 / BNT]) {
+  constexpr int CH = R * ROWB / 16 / BNT;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int id = threadIdx.x + c * BNT;
+    const int r = id >> 2, q = id & 3;
+    *reinterpret_cast<ux4*>(S + swz(r, q)) = reg[c];
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
+  constexpr int NPART = MODE == M_BF16X3 ? 2 : 1;
+  constexpr int EB = MODE == M_BF16X3 ? 2 : 4;
+  constexpr int WM = 128, WN = 64;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int CH = BIG * ROWB / 16 / BNT;   // 16-B chunks per thread per operand part
+
+  __shared__ __attribute__((aligned(16))) char As[2][NPART][BIG * ROWB];
+  __shared__ __attribute__((aligned(16))) char Bs[2][NPART][BIG * ROWB];
+
+  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
+  int pi = 0;
+  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
+  const GemmProb& P = batch.p[pi];
+  const int u = t - P.tile_start;
+  const int per_group = GROUP_M * P.tiles_n;
+  const int g = u / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsize = min(GROUP_M, P.tiles_m - first_m);
+  const int tm = first_m + (u % per_group) % gsize;
+  const int tn = (u % per_group) / gsize;
+  const int m0 = tm * BIG, n0 = tn * BIG;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  fx4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+
+  constexpr int KSTEP = ROWB / EB;
+  int nsteps = 0;
+  for (int s = 0; s < P.nseg; ++s) nsteps += (P.seg[s].K + KSTEP - 1) / KSTEP;
+
+  ux4 ra[NPART][CH], rb[NPART][CH];
+  auto load = [&](int s, int k0) {
+    const GemmSeg& S = P.seg[s];
+    const int64_t ldab = S.lda * EB, ldbb = S.ldb * EB;
+    const int64_t kb0 = (int64_t)k0 * EB, Kb = (int64_t)S.K * EB;
+    load_slab_big<BIG>(static_cast<const char*>(S.A), ldab, m0, kb0, P.M, Kb, ra[0]);
+    load_slab_big<BIG>(static_cast<const char*>(S.B), ldbb, n0, kb0, P.N, Kb, rb[0]);
+    if (NPART == 2) {
+      load_slab_big<BIG>(static_cast<const char*>(S.Alo), ldab, m0, kb0, P.M, Kb, ra[NPART - 1]);
+      load_slab_big<BIG>(static_cast<const char*>(S.Blo), ldbb, n0, kb0, P.N, Kb, rb[NPART - 1]);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NPART; ++p) {
+      store_slab_big<BIG>(As[buf][p], ra[p]);
+      store_slab_big<BIG>(Bs[buf][p], rb[p]);
+    }
+  };
+
+  int s = 0, k0 = 0;
+  if (nsteps > 0) {
+    load(0, 0);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  const int aoff = swz(wm0 + li, lg);   // same swizzle for every 16-row block
+  const int boff = swz(wn0 + li, lg);
+  for (int step = 0; step < nsteps; ++step) {
+    int ns = s, nk0 = k0 + KSTEP;
+    if (nk0 >= P.seg[s].K) { ns = s + 1; nk0 = 0; }
+    const bool has_next = step + 1 < nsteps;
+    if (has_next) load(ns, nk0);
+
+    ux4 fb[NPART][FN];
+#pragma unroll
+    for (int p = 0; p < NPART; ++p)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[p][j] = *reinterpret_cast<const ux4*>(Bs[buf][p] + boff + j * 16 * ROWB);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      ux4 fa[NPART];
+#pragma unroll
+      for (int p = 0; p < NPART; ++p) fa[p] = *reinterpret_cast<const ux4*>(As[buf][p] + aoff + i * 16 * ROWB);
+      if constexpr (MODE == M_BF16X3) {
+        const bfx8 ah = __builtin_bit_cast(bfx8, fa[0]);
+        const bfx8 al = __builtin_bit_cast(bfx8, fa[1]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const bfx8 bh = __builtin_bit_cast(bfx8, fb[0][j]);
+          const bfx8 bl = __builtin_bit_cast(bfx8, fb[1][j]);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[i][j], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int sub = 0; sub < 4; ++sub) {
+          const unsigned int ua = fa[0][sub];
+          const float a = __uint_as_float(ua);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            const unsigned int ub = fb[0][j][sub];
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, __uint_as_float(ub), acc[i][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    if (has_next) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+    s = ns;
+    k0 = nk0;
+  }
+
+  const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn0 + j * 16 + li;
+      if (col >= P.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm0 + i * 16 + lg * 4 + r;
+        if (row >= P.M) continue;
+        double v = (double)acc[i][j][r];
+        if (P.epi == EPI_CB) v += cb_term(P.cb, P.cbx, P.Dl, P.Dr, P.d1, P.d2, row, col);
+        double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
+        v *= alpha;
+        if (beta != 0.0) v += beta * *cp;
+        *cp = v;
+      }
+    }
+  }
+}
+
+template <int BM, int BN, int MODE, bool BIGK = false>
 mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
   constexpr int EB = MODE == M_BF16X3 ? 2 : 4;
   GemmBatch b{};
@@ -263,7 +450,10 @@ mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
   b.total_tiles = total;
   if (total == 0) return MK_OK;
   ProfScope prof(MODE == M_BF16X3 ? "gemm_bf16x3" : "gemm_f32", stream);
-  hipLaunchKernelGGL((gemm_mixed_kernel<BM, BN, MODE>), dim3(total), dim3(NT), 0, stream, b);
+  if constexpr (BIGK)
+    hipLaunchKernelGGL((gemm_mixed_big_kernel<MODE>), dim3(total), dim3(BNT), 0, stream, b);
+  else
+    hipLaunchKernelGGL((gemm_mixed_kernel<BM, BN, MODE>), dim3(total), dim3(NT), 0, stream, b);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
 }
@@ -276,6 +466,10 @@ mk_status dispatch(const ProbSpec* probs, int nprob, hipStream_t stream) {
     MK_REQUIRE(probs[i].nseg <= MAX_SEG, MK_BAD_ARG, "mixed gemm: too many segments");
     tiles128 += cdiv(probs[i].M, 128) * cdiv(probs[i].N, 128);
   }
+  int64_t tiles256 = 0;
+  for (int i = 0; i < nprob; ++i) tiles256 += cdiv(probs[i].M, 256) * cdiv(probs[i].N, 256);
+  // 256x256 tiles once they alone fill the 256 CUs; 128x128 while they fill them; else 64x64
+  if (tiles256 >= 256 && !g_force_small) return launch_mixed<256, 256, MODE, true>(probs, nprob, stream);
   if (tiles128 >= 256) return launch_mixed<128, 128, MODE>(probs, nprob, stream);
   return launch_mixed<64, 64, MODE>(probs, nprob, stream);
 }

@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = {"fp64": 1e-13, "fp32": 1e-5, "bf16x3": 3e-4}
 SHAPES = [(1, 16, 16), (16, 16, 16), (64, 64, 64), (67, 45, 131), (128, 256, 96), (300, 257, 129),
-          (2048, 2048, 64), (2176, 2048, 200)]
+          (2048, 2048, 64), (2176, 2048, 200),
+          (4096, 4096, 96), (4200, 4100, 130)]       # the last two take the 256x256 mixed kernel
 
 
 @pytest.fixture(scope="module")
