@@ -151,6 +151,8 @@ def main():
         if prec == "fp64":
             y64 = y.clone()
             err = 0.0
+        elif y64 is None:
+            err = float("nan")
         else:
             err = float(torch.linalg.vector_norm(y - y64) / torch.linalg.vector_norm(y64))
         res[prec] = dict(ms=ms, gemm_ms=gemm_ms, launches=cnt.value, err=err)
