@@ -82,7 +82,16 @@ mk_status carve(Ctx* ctx, std::initializer_list<std::pair<size_t, void**>> regio
 // Matrices with optional mixed-precision copies
 // ---------------------------------------------------------------------------
 
-int64_t pad8(int64_t k) { return (k + 7) / 8 * 8; }
+// Row length (elements) of a K-contiguous mixed-precision operand: K rounded up
+// to 8 (16-byte rows), plus 64 when that is a multiple of 128 and at least 512,
+// so that long rows are not a multiple of a large power of two apart; the rows
+// of one K slab then spread over the L2 and HBM channels instead of camping
+// on one.  The padding is zero.
+int64_t pad8(int64_t k) {
+  int64_t p = (k + 7) / 8 * 8;
+  if (p >= 512 && p % 128 == 0) p += 64;
+  return p;
+}
 
 // A float64 column-major operator matrix (rows x cols) used as GEMM operand
 // `role` ('A' = left factor, rows = M, cols = K;  'B' = right factor,

@@ -4,7 +4,7 @@
 // column-major matrix products with free leg fusions (reshapes), so one GEMM
 // family serves all of them:
 //
-//   C_p (M_p x N_p) = alpha_p * sum_s op(A_ps) op(B_ps) + beta_p * C_p  (+ epilogue)
+//   C_p (M_p x N_p) = alpha_p * sum_s op(A_ps) op(B_ps) + beta_p * C_p
 //
 // A launch holds up to MAX_PROB independent problems (e.g. the d output
 // blocks of a strided term) and each problem sums up to MAX_SEG segments
@@ -17,13 +17,6 @@ namespace mk {
 
 constexpr int MAX_SEG = 4;
 constexpr int MAX_PROB = 4;
-
-enum EpiKind : int32_t {
-  EPI_NONE = 0,
-  // two-site "starting left - ending right" term of the Jordan apply
-  // (hamiltonian_derivatives.jl:491): y[a,b,c,e] += CB[b,e,h,j] x[a,h,c,j]
-  EPI_CB = 1,
-};
 
 struct GemmSeg {
   const void* A;     // fp64 operand, or the hi part (bf16) / fp32 copy
@@ -43,10 +36,6 @@ struct GemmProb {
   double* C;
   int64_t ldc;
   double alpha, beta;
-  int32_t epi;
-  int32_t Dl, Dr, d1, d2;
-  const double* cb;   // CB[b,e,h,j]  (d1 x d2 x d1 x d2), column-major
-  const double* cbx;  // x[a,h,c,j]   (Dl x d1 x Dr x d2)
 };
 
 struct GemmBatch {
@@ -74,10 +63,6 @@ struct ProbSpec {
   double* C = nullptr;
   int64_t ldc = 0;
   double alpha = 1.0, beta = 0.0;
-  int32_t epi = EPI_NONE;
-  int32_t Dl = 0, Dr = 0, d1 = 0, d2 = 0;
-  const double* cb = nullptr;
-  const double* cbx = nullptr;
   void add(const void* A, int64_t lda, const void* B, int64_t ldb, int64_t K,
            const void* Alo = nullptr, const void* Blo = nullptr) {
     seg[nseg].A = A; seg[nseg].lda = lda; seg[nseg].B = B; seg[nseg].ldb = ldb;

@@ -1,5 +1,5 @@
 // FP64 MFMA GEMM for CDNA4 (gfx950): v_mfma_f64_16x16x4_f64, LDS-staged
-// BK=16 tiles, double-buffered through registers, 256 threads = 4 waves of 64
+// BK=32 tiles, double-buffered through registers, 256 threads = 4 waves of 64
 // lanes in a 2x2 arrangement.  One workgroup computes one BM x BN output tile
 // of one problem, summing every segment of that problem into the same
 // accumulators before a single fused epilogue (alpha/beta and optional CB term).
@@ -67,7 +67,13 @@ __device__ __forceinline__ void store_fastdir(double (*S)[LDR], const dx2 (&reg)
 }
 
 // Stage a tile whose contiguous memory direction is k.
-// element (r, k) lives at p[(k0 + k) + (r0 + r) * ld]; LDS tile S[k][r].
+// element (r, k) lives at p[(k0 + k) + (r0 + r) * ld]; LDS tile S[k][r ^ kswz(k)].
+// The 16 lanes that share a column write rows k = 0, 2, ..., 30; with a row
+// length that is a multiple of 32 doubles they would all hit one bank pair, so
+// the column index is XORed with bits 1..4 of k (a permutation inside each
+// aligned 32-column group, undone by the fragment reads).
+__device__ __forceinline__ int kswz(int k) { return k & 30; }
+
 template <int R, int LDR>
 __device__ __forceinline__ void load_kdir(const double* __restrict__ p, int64_t ld, int r0, int k0,
                                           int Rlim, int Klim, bool vec,
@@ -101,8 +107,8 @@ __device__ __forceinline__ void store_kdir(double (*S)[LDR], const dx2 (&reg)[R 
     const int id = threadIdx.x + c * NT;
     const int r = id / (BK / 2);
     const int k = (id % (BK / 2)) * 2;
-    S[k][r] = reg[c].x;
-    S[k + 1][r] = reg[c].y;
+    S[k][r ^ kswz(k)] = reg[c].x;
+    S[k + 1][r ^ kswz(k)] = reg[c].y;
   }
 }
 
@@ -183,15 +189,18 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       double a[FM], b[FN];
+      const int kr = kk * 4 + lk;
+      const int sa = TA ? kswz(kr) : 0;    // K-contiguous operands were stored swizzled
+      const int sb = TB ? 0 : kswz(kr);
 #pragma unroll
-      for (int i = 0; i < FM; ++i) a[i] = As[buf][kk * 4 + lk][wm0 + i * 16 + li];
+      for (int i = 0; i < FM; ++i) a[i] = As[buf][kr][(wm0 + i * 16 + li) ^ sa];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) b[j] = Bs[buf][kk * 4 + lk][wn0 + j * 16 + li];
+      for (int j = 0; j < FN; ++j) b[j] = Bs[buf][kr][(wn0 + j * 16 + li) ^ sb];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
     }
     if (has_next) store(buf ^ 1);
     __syncthreads();
@@ -200,32 +209,23 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
     k0 = nk0;
   }
 
-  // epilogue: v_mfma_f64_16x16x4f64 D lane layout -- lane (li, lk) item r holds
-  // row lk + 4*r, column li (unlike the f32/bf16 16x16 MFMAs, rows 4*lk + r)
+  // epilogue.  The v_mfma_f64_16x16x4f64 D layout puts item r of lane (li, lk)
+  // at block row lk + 4 r, column li; with the B fragment as first operand the
+  // block is C^T, so the lane holds C[row li][col lk + 4 r]: the 16 lanes of a
+  // quarter-wave store 16 consecutive rows (128 B) of one column.
   const double alpha = P.alpha, beta = P.beta;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
+    const int row = m0 + wm0 + i * 16 + li;
+    if (row >= P.M) continue;
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn0 + j * 16 + li;
-      if (col >= P.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + lk + 4 * r;
-        if (row >= P.M) continue;
-        double v = acc[i][j][r];
-        if (P.epi == EPI_CB) {
-          const int a = row % P.Dl, bb = row / P.Dl;
-          const int c = col % P.Dr, e = col / P.Dr;
-          double tsum = 0.0;
-          for (int jj = 0; jj < P.d2; ++jj)
-            for (int h = 0; h < P.d1; ++h)
-              tsum += P.cb[bb + P.d1 * (e + P.d2 * (h + P.d1 * jj))] *
-                      P.cbx[a + (int64_t)P.Dl * (h + P.d1 * ((int64_t)c + (int64_t)P.Dr * jj))];
-          v += tsum;
-        }
+        const int col = n0 + wn0 + j * 16 + lk + 4 * r;
+        if (col >= P.N) continue;
         double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
-        v *= alpha;
+        double v = alpha * acc[i][j][r];
         if (beta != 0.0) v += beta * *cp;
         *cp = v;
       }
@@ -264,10 +264,6 @@ mk_status launch_cfg(const ProbSpec* probs, int nprob, GemmShape sh, hipStream_t
     P.ldc = ps.ldc;
     P.alpha = ps.alpha;
     P.beta = ps.beta;
-    P.epi = ps.epi;
-    P.Dl = ps.Dl; P.Dr = ps.Dr; P.d1 = ps.d1; P.d2 = ps.d2;
-    P.cb = ps.cb;
-    P.cbx = ps.cbx;
   }
   b.nprob = nprob;
   b.total_tiles = total;

@@ -14,6 +14,9 @@
 // by A and B.
 #include <hip/hip_bf16.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "gemm.h"
 #include "prof.h"
 
@@ -32,7 +35,18 @@ constexpr int GROUP_M = 8;
 
 enum Mode { M_BF16X3 = 0, M_F32 = 1 };
 
-bool g_force_small = false;  // test hook: keep the 128/64 kernels for every shape
+// Kernel family for large problems, read once from MK_GEMM_VARIANT ("big",
+// "pipe", "tile"; default "big"): lets benchmarks and tests pin one family.
+enum Variant { V_PIPE = 0, V_BIG = 1, V_TILE = 2 };
+int variant() {
+  static const int v = [] {
+    const char* e = getenv("MK_GEMM_VARIANT");
+    if (e && !strcmp(e, "pipe")) return (int)V_PIPE;
+    if (e && !strcmp(e, "tile")) return (int)V_TILE;
+    return (int)V_BIG;
+  }();
+  return v;
+}
 
 __device__ __forceinline__ int xcd_remap(int t, int total) {
   const int q = total / NXCD, r = total % NXCD;
@@ -40,18 +54,30 @@ __device__ __forceinline__ int xcd_remap(int t, int total) {
   return x * q + min(x, r) + local;
 }
 
-// Two-site physical (CB) term of one output element y[a,b,c,e] (row = a + Dl b,
-// col = c + Dr e): sum_{h,j} CB[b,e,h,j] x[a,h,c,j].  Outlined with scalar
-// arguments so the unrolled epilogues stay small.
-__device__ __noinline__ double cb_term(const double* __restrict__ cb, const double* __restrict__ x, int Dl, int Dr,
-                                       int d1, int d2, int row, int col) {
-  const int a = row % Dl, bb = row / Dl;
-  const int c = col % Dr, e = col / Dr;
-  double t = 0.0;
-  for (int jj = 0; jj < d2; ++jj)
-    for (int h = 0; h < d1; ++h)
-      t += cb[bb + d1 * (e + d2 * (h + d1 * jj))] * x[a + (int64_t)Dl * (h + d1 * ((int64_t)c + (int64_t)Dr * jj))];
-  return t;
+// Epilogue of an FM x FN grid of 16x16 fp32 accumulator blocks.  The MFMAs run
+// with the B fragment as first operand, so block (i, j) is C^T and item r of
+// lane (li, lg) is C[row0 + 16 i + li][col0 + 16 j + 4 lg + r]: the 16 lanes of
+// a quarter-wave store 16 consecutive rows (128 B) of one column.
+template <int FM, int FN>
+__device__ __forceinline__ void store_tile(const fx4 (&acc)[FM][FN], double* __restrict__ C, int64_t ldc, int M,
+                                           int N, int row0, int col0, int li, int lg, double alpha, double beta) {
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int row = row0 + i * 16 + li;
+    if (row >= M) continue;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = col0 + j * 16 + lg * 4 + r;
+        if (col >= N) continue;
+        double* cp = C + (int64_t)row + (int64_t)col * ldc;
+        double v = alpha * (double)acc[i][j][r];
+        if (beta != 0.0) v += beta * *cp;
+        *cp = v;
+      }
+    }
+  }
 }
 
 // Load a (R rows x 64 B) slab of a K-contiguous operand into registers.
@@ -181,9 +207,9 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
           const bfx8 al = __builtin_bit_cast(bfx8, fa[1][i]);
           const bfx8 bh = __builtin_bit_cast(bfx8, fb[0][j]);
           const bfx8 bl = __builtin_bit_cast(bfx8, fb[1][j]);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah, acc[i][j], 0, 0, 0);
         }
     } else {
 #pragma unroll
@@ -197,7 +223,7 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
             const unsigned int ua = fa[0][i][sub], ub = fb[0][j][sub];
             const float a = __uint_as_float(ua);
             const float b = __uint_as_float(ub);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, acc[i][j], 0, 0, 0);
           }
     }
     if (has_next) store(buf ^ 1);
@@ -207,26 +233,7 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
     k0 = nk0;
   }
 
-  const double alpha = P.alpha, beta = P.beta;
-#pragma unroll
-  for (int i = 0; i < FM; ++i) {
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn0 + j * 16 + li;
-      if (col >= P.N) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + lg * 4 + r;
-        if (row >= P.M) continue;
-        double v = (double)acc[i][j][r];
-        if (P.epi == EPI_CB) v += cb_term(P.cb, P.cbx, P.Dl, P.Dr, P.d1, P.d2, row, col);
-        double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
-        v *= alpha;
-        if (beta != 0.0) v += beta * *cp;
-        *cp = v;
-      }
-    }
-  }
+  store_tile<FM, FN>(acc, P.C, P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
 }
 
 // ---------------------------------------------------------------------------
@@ -363,9 +370,9 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
         for (int j = 0; j < FN; ++j) {
           const bfx8 bh = __builtin_bit_cast(bfx8, fb[0][j]);
           const bfx8 bl = __builtin_bit_cast(bfx8, fb[1][j]);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah, acc[i][j], 0, 0, 0);
         }
       } else {
 #pragma unroll
@@ -375,7 +382,7 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
             const unsigned int ub = fb[0][j][sub];
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, __uint_as_float(ub), acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ub), a, acc[i][j], 0, 0, 0);
           }
         }
       }
@@ -387,29 +394,183 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
     k0 = nk0;
   }
 
-  const double alpha = P.alpha, beta = P.beta;
+  store_tile<FM, FN>(acc, P.C, P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined variant for large problems: 256 x 128 output tile, 256 threads =
+// 4 waves (2 x 2) of 128 x 64, three LDS stages.  Operand slabs go global ->
+// LDS through buffer_load ... lds (no staging registers), so the loads of step
+// s + 2 are in flight while step s computes; a single-stage register prefetch
+// covers less than one L2-miss latency at this MFMA rate.  Lane l of a wave's
+// 16-byte LDS load writes LDS bytes [16 l, 16 l + 16) after the wave base, so
+// the swizzle is applied on the global side: LDS chunk qp of row r receives
+// logical chunk qp ^ ((r >> 2) & 3).  Out-of-range rows and K chunks get a
+// buffer offset past num_records, which the hardware turns into zeros.
+// Three stages of hi + lo bf16 slabs: 3 x (32 + 16) KB = 144 KB of LDS.
+// ---------------------------------------------------------------------------
+
+constexpr int PBM = 256, PBN = 128, PNT = 256, NSTAGE = 3;
+constexpr unsigned OOB = 0x80000000u;
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p) {
+  // raw (stride 0) buffer, num_records = 2^31 - 1 bytes, gfx9 data format word
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+
+// s_waitcnt on vmcnt only (gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14])
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Workgroup barrier without the release fence of __syncthreads(): that fence
+// would wait for every outstanding LDS-DMA load (vmcnt(0)) and serialise the
+// pipeline.  Each wave waits for its own loads of the stage it is about to read
+// (wait_vm) before the barrier, and every LDS read of the previous stage has
+// been consumed by an MFMA before it, so the barrier alone orders the stages.
+// The empty asm statements keep the compiler from moving LDS accesses across.
+__device__ __forceinline__ void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// One R x 64 B slab of a K-contiguous operand into LDS stage memory S.
+template <int R>
+__device__ __forceinline__ void dma_slab(__amdgpu_buffer_rsrc_t rs, char* S, int ld_bytes, int r0, int Rlim,
+                                         int kb0, int Kbytes, int wave) {
+  constexpr int CH = R * (ROWB / 16) / PNT;
+  const int t = threadIdx.x;
 #pragma unroll
-  for (int i = 0; i < FM; ++i) {
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn0 + j * 16 + li;
-      if (col >= P.N) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + lg * 4 + r;
-        if (row >= P.M) continue;
-        double v = (double)acc[i][j][r];
-        if (P.epi == EPI_CB) v += cb_term(P.cb, P.cbx, P.Dl, P.Dr, P.d1, P.d2, row, col);
-        double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
-        v *= alpha;
-        if (beta != 0.0) v += beta * *cp;
-        *cp = v;
-      }
-    }
+  for (int c = 0; c < CH; ++c) {
+    const int idx = c * PNT + t;
+    const int r = idx >> 2, qp = idx & 3;
+    const int q = qp ^ ((r >> 2) & 3);
+    const int gr = r0 + r, kb = kb0 + q * 16;
+    const unsigned vo = (gr < Rlim && kb < Kbytes) ? (unsigned)(gr * ld_bytes + kb) : OOB;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(S + (c * PNT + wave * 64) * 16), 16, vo, 0, 0, 0);
   }
 }
 
-template <int BM, int BN, int MODE, bool BIGK = false>
+template <int MODE>
+__global__ __launch_bounds__(PNT) void gemm_mixed_pipe_kernel(GemmBatch batch) {
+  constexpr int NPART = MODE == M_BF16X3 ? 2 : 1;
+  constexpr int EB = MODE == M_BF16X3 ? 2 : 4;
+  constexpr int WM = 128, WN = 64;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int KSTEP = ROWB / EB;
+  constexpr int ABYTES = PBM * ROWB, BBYTES = PBN * ROWB;
+  constexpr int SBYTES = NPART * (ABYTES + BBYTES);
+  // loads issued per thread per stage (each counts once in vmcnt)
+  constexpr int LPS = NPART * (PBM + PBN) * (ROWB / 16) / PNT;
+
+  __shared__ __attribute__((aligned(16))) char smem[NSTAGE * SBYTES];
+
+  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
+  int pi = 0;
+  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
+  const GemmProb& P = batch.p[pi];
+  const int u = t - P.tile_start;
+  const int per_group = GROUP_M * P.tiles_n;
+  const int g = u / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsize = min(GROUP_M, P.tiles_m - first_m);
+  const int tm = first_m + (u % per_group) % gsize;
+  const int tn = (u % per_group) / gsize;
+  const int m0 = tm * PBM, n0 = tn * PBN;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  fx4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+
+  int nsteps = 0;
+  for (int s = 0; s < P.nseg; ++s) nsteps += (P.seg[s].K + KSTEP - 1) / KSTEP;
+
+  // load cursor (segment, k) of the next stage to fetch
+  int ls = 0, lk = 0;
+  auto fetch = [&](int stage) {
+    const GemmSeg& S = P.seg[ls];
+    const int ldab = (int)S.lda * EB, ldbb = (int)S.ldb * EB;
+    const int kb0 = lk * EB, Kb = S.K * EB;
+    char* base = smem + stage * SBYTES;
+    dma_slab<PBM>(raw_rsrc(S.A), base, ldab, m0, P.M, kb0, Kb, wave);
+    dma_slab<PBN>(raw_rsrc(S.B), base + NPART * ABYTES, ldbb, n0, P.N, kb0, Kb, wave);
+    if constexpr (NPART == 2) {
+      dma_slab<PBM>(raw_rsrc(S.Alo), base + ABYTES, ldab, m0, P.M, kb0, Kb, wave);
+      dma_slab<PBN>(raw_rsrc(S.Blo), base + 2 * ABYTES + BBYTES, ldbb, n0, P.N, kb0, Kb, wave);
+    }
+    lk += KSTEP;
+    if (lk >= S.K) { ++ls; lk = 0; }
+  };
+
+  if (nsteps > 0) fetch(0);
+  if (nsteps > 1) fetch(1);
+  const int aoff = swz(wm0 + li, lg);   // same swizzle for every 16-row block
+  const int boff = swz(wn0 + li, lg);
+  int stage = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    // this wave's loads of `step` are done once at most one newer stage is pending
+    if (step + 1 < nsteps) wait_vm<LPS>();
+    else wait_vm<0>();
+    stage_barrier();   // every wave's loads of `step` landed; stage (step + 2) % 3 is free
+    if (step + 2 < nsteps) fetch(stage == 0 ? 2 : stage - 1);
+
+    const char* As = smem + stage * SBYTES;
+    const char* Bs = As + NPART * ABYTES;
+    ux4 fb[NPART][FN];
+#pragma unroll
+    for (int p = 0; p < NPART; ++p)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[p][j] = *reinterpret_cast<const ux4*>(Bs + p * BBYTES + boff + j * 16 * ROWB);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      ux4 fa[NPART];
+#pragma unroll
+      for (int p = 0; p < NPART; ++p) fa[p] = *reinterpret_cast<const ux4*>(As + p * ABYTES + aoff + i * 16 * ROWB);
+      if constexpr (MODE == M_BF16X3) {
+        const bfx8 ah = __builtin_bit_cast(bfx8, fa[0]);
+        const bfx8 al = __builtin_bit_cast(bfx8, fa[1]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const bfx8 bh = __builtin_bit_cast(bfx8, fb[0][j]);
+          const bfx8 bl = __builtin_bit_cast(bfx8, fb[1][j]);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah, acc[i][j], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int sub = 0; sub < 4; ++sub) {
+          const unsigned int ua = fa[0][sub];
+          const float a = __uint_as_float(ua);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            const unsigned int ub = fb[0][j][sub];
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ub), a, acc[i][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    stage = stage == NSTAGE - 1 ? 0 : stage + 1;
+  }
+
+  store_tile<FM, FN>(acc, P.C, P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
+}
+
+template <int BM, int BN, int MODE, int VAR>
 mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
   constexpr int EB = MODE == M_BF16X3 ? 2 : 4;
   GemmBatch b{};
@@ -442,15 +603,15 @@ mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
     P.tile_start = total;
     total += P.tiles_m * P.tiles_n;
     P.C = ps.C; P.ldc = ps.ldc; P.alpha = ps.alpha; P.beta = ps.beta;
-    P.epi = ps.epi; P.Dl = ps.Dl; P.Dr = ps.Dr; P.d1 = ps.d1; P.d2 = ps.d2;
-    P.cb = ps.cb; P.cbx = ps.cbx;
   }
   (void)EB;
   b.nprob = nprob;
   b.total_tiles = total;
   if (total == 0) return MK_OK;
   ProfScope prof(MODE == M_BF16X3 ? "gemm_bf16x3" : "gemm_f32", stream);
-  if constexpr (BIGK)
+  if constexpr (VAR == V_PIPE)
+    hipLaunchKernelGGL((gemm_mixed_pipe_kernel<MODE>), dim3(total), dim3(PNT), 0, stream, b);
+  else if constexpr (VAR == V_BIG)
     hipLaunchKernelGGL((gemm_mixed_big_kernel<MODE>), dim3(total), dim3(BNT), 0, stream, b);
   else
     hipLaunchKernelGGL((gemm_mixed_kernel<BM, BN, MODE>), dim3(total), dim3(NT), 0, stream, b);
@@ -466,12 +627,22 @@ mk_status dispatch(const ProbSpec* probs, int nprob, hipStream_t stream) {
     MK_REQUIRE(probs[i].nseg <= MAX_SEG, MK_BAD_ARG, "mixed gemm: too many segments");
     tiles128 += cdiv(probs[i].M, 128) * cdiv(probs[i].N, 128);
   }
-  int64_t tiles256 = 0;
-  for (int i = 0; i < nprob; ++i) tiles256 += cdiv(probs[i].M, 256) * cdiv(probs[i].N, 256);
-  // 256x256 tiles once they alone fill the 256 CUs; 128x128 while they fill them; else 64x64
-  if (tiles256 >= 256 && !g_force_small) return launch_mixed<256, 256, MODE, true>(probs, nprob, stream);
-  if (tiles128 >= 256) return launch_mixed<128, 128, MODE>(probs, nprob, stream);
-  return launch_mixed<64, 64, MODE>(probs, nprob, stream);
+  constexpr int64_t EB = MODE == M_BF16X3 ? 2 : 4;
+  int64_t tiles256 = 0, tilesp = 0;
+  bool off32 = true;   // buffer offsets of the pipelined kernel are 32-bit
+  for (int i = 0; i < nprob; ++i) {
+    tiles256 += cdiv(probs[i].M, 256) * cdiv(probs[i].N, 256);
+    tilesp += cdiv(probs[i].M, PBM) * cdiv(probs[i].N, PBN);
+    for (int s = 0; s < probs[i].nseg; ++s)
+      off32 = off32 && probs[i].M * probs[i].seg[s].lda * EB < (1ll << 31) &&
+              probs[i].N * probs[i].seg[s].ldb * EB < (1ll << 31);
+  }
+  const int var = variant();
+  // large tiles once they alone fill the 256 CUs; 128x128 while they fill them; else 64x64
+  if (var == V_PIPE && tilesp >= 256 && off32) return launch_mixed<PBM, PBN, MODE, V_PIPE>(probs, nprob, stream);
+  if (var == V_BIG && tiles256 >= 256) return launch_mixed<256, 256, MODE, V_BIG>(probs, nprob, stream);
+  if (tiles128 >= 256) return launch_mixed<128, 128, MODE, V_TILE>(probs, nprob, stream);
+  return launch_mixed<64, 64, MODE, V_TILE>(probs, nprob, stream);
 }
 
 }  // namespace

@@ -224,6 +224,38 @@ __global__ void phys2_kernel(const double* __restrict__ P, const double* __restr
   }
 }
 
+// Same contraction with one thread per (a, c): the d1*d2 inputs x[a,:,c,:] are
+// read once into registers and all d1*d2 outputs y[a,:,c,:] written from them,
+// so x is read once and y written once (loads and stores coalesced along a).
+template <int D12>
+__global__ void phys2_reg_kernel(const double* __restrict__ P, const double* __restrict__ x, int64_t Dl,
+                                 int d1, int64_t Dr, int d2, double alpha, double beta,
+                                 double* __restrict__ y) {
+  const int64_t n = Dl * Dr;
+  const int64_t sx = Dl * d1 * Dr;   // stride of the second physical leg
+  GRID_STRIDE(idx, n) {
+    const int64_t a = idx % Dl, c = idx / Dl;
+    const int64_t base = a + Dl * d1 * c;
+    double xv[D12];
+#pragma unroll
+    for (int t = 0; t < D12; ++t) {
+      const int h = t % d1, j = t / d1;
+      xv[t] = x[base + Dl * h + sx * j];
+    }
+#pragma unroll
+    for (int o = 0; o < D12; ++o) {
+      const int b = o % d1, e = o / d1;
+      double s = 0.0;
+#pragma unroll
+      for (int t = 0; t < D12; ++t) s += P[b + d1 * (e + d2 * t)] * xv[t];
+      double* yp = y + base + Dl * b + sx * e;
+      double v = alpha * s;
+      if (beta != 0.0) v += beta * *yp;
+      *yp = v;
+    }
+  }
+}
+
 __global__ void phys1_kernel(const double* __restrict__ P, const double* __restrict__ x, int64_t Dl,
                              int d, int64_t Dr, double alpha, double beta, double* __restrict__ y) {
   const int64_t n = Dl * d * Dr;
@@ -478,7 +510,12 @@ mk_status k_tr_mid(const double* t1, const double* O, int64_t Da, int wl, int d,
 }
 mk_status k_phys2(const double* P, const double* x, int64_t Dl, int d1, int64_t Dr, int d2,
                   double alpha, double beta, double* y, hipStream_t st) {
-  LAUNCH_1D(phys2_kernel, Dl * d1 * Dr * d2, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
+  switch (d1 * d2) {
+    case 4: LAUNCH_1D(phys2_reg_kernel<4>, Dl * Dr, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
+    case 9: LAUNCH_1D(phys2_reg_kernel<9>, Dl * Dr, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
+    case 16: LAUNCH_1D(phys2_reg_kernel<16>, Dl * Dr, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
+    default: LAUNCH_1D(phys2_kernel, Dl * d1 * Dr * d2, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
+  }
 }
 mk_status k_phys1(const double* P, const double* x, int64_t Dl, int d, int64_t Dr, double alpha,
                   double beta, double* y, hipStream_t st) {

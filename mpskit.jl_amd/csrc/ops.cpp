@@ -214,6 +214,10 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
     if (need_ca) MK_TRY(convert_view(ctx, x, Dl, d1 * N, Dl, 'A', prec, &xca, mc));
   }
   const GemmShape nn{};
+  // two-site CB term first: a memory-bound pass that reads x once
+  if (op->hasP)
+    MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, alpha, bs.next(), y,
+                   ctx->stream));
   if (op->IC.present() || op->BE.present()) {
     ProbSpec p;
     p.M = M; p.N = N; p.C = y; p.ldc = M; p.alpha = alpha; p.beta = bs.next();
@@ -226,12 +230,6 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
       Side a = mat_side(op->BE, prec);
       Side b = prec == MK_FP64 ? Side{x, nullptr, M} : view_side(xb);
       p.add(a.p, a.ld, b.p, b.ld, M, a.lo, b.lo);
-    }
-    if (op->hasP) {
-      p.epi = EPI_CB;
-      p.Dl = (int32_t)Dl; p.Dr = (int32_t)Dr; p.d1 = (int32_t)d1; p.d2 = (int32_t)d2;
-      p.cb = static_cast<const double*>(op->P.ptr);
-      p.cbx = x;
     }
     MK_TRY(run_gemm(&p, 1, prec, nn, ctx->stream));
   }
@@ -263,9 +261,6 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
     MK_TRY(run_gemm(ps, (int)d2, prec, nn, ctx->stream));
   }
   if (op->gen.present()) MK_TRY(apply_lxr(op, op->gen, x, y, alpha, bs.next(), prec));
-  if (op->hasP && !(op->IC.present() || op->BE.present()))
-    MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, alpha, bs.next(), y,
-                   ctx->stream));
   if (bs.first) MK_TRY(k_axpby(0.0, y, beta, y, op->y_elems, ctx->stream));
   return MK_OK;
 }
