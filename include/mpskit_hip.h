@@ -203,6 +203,38 @@ mk_status mk_gemm(mk_ctx ctx, int transa, int transb, int64_t M, int64_t N, int6
                   double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                   double beta, double* C, int64_t ldc, int precision);
 
+/* ---- batches -----------------------------------------------------------------
+ * Counterpart of `mk_apply_batched(ops[], xs[], ys[], n)` and of running the
+ * local eigensolves of many instances at once: the reference applies one
+ * derivative operator per call (derivatives.jl:8-11) and solves one local
+ * problem per fixedpoint call (fixedpoint.jl:15-18); callers that sweep many
+ * instances in lockstep (parameter sweeps, symmetry sectors) bind these.
+ *
+ * A batch groups n prepared square operators of one context (they must stay
+ * alive while the batch exists).  Vectors of a batch are segmented: instance
+ * i occupies elements [offsets[i], offsets[i] + x_elems_i) of one device
+ * buffer of `total` doubles; offsets are multiples of 4096 and the padding
+ * between instances must be zero (it is never written).  When every operator
+ * is a nearest-neighbour Jordan two-site operator (`grouped` = 1) an apply of
+ * the whole batch is one grouped persistent GEMM launch plus two grouped
+ * memory-bound passes; otherwise operators are applied one by one.  Tile
+ * shapes of grouped launches are chosen by the library's autotuner. */
+typedef struct mk_batch_s* mk_batch;
+mk_status mk_batch_create(mk_ctx ctx, int n, const mk_op* ops, mk_batch* out);
+mk_status mk_batch_layout(mk_batch b, int64_t* offsets, int64_t* total, int* grouped);
+/* y = H x blockwise (padding of y not written). */
+mk_status mk_batch_apply(mk_batch b, const double* x, double* y, int precision);
+/* Lowest eigenpair of every instance (mk_eigsolve's algorithm in lockstep);
+ * x holds the start vectors and receives the normalised eigenvectors.
+ * use_graph = 1 captures each Krylov step's device work once into a hipGraph
+ * and replays it (grouped batches only).  Outputs are arrays of length n. */
+mk_status mk_batch_eigsolve(mk_batch b, double* x, int krylovdim, int maxiter, double tol, int precision,
+                            int use_graph, double* eigvals, double* resnorms, int* numops, int* converged);
+mk_status mk_batch_free(mk_batch b);
+/* Autotuner decisions taken and cache hits so far in this process (no
+ * counterpart in the reference). */
+mk_status mk_autotune_stats(int* decisions, int* hits);
+
 /* ---- kernel timing ---------------------------------------------------------
  * No counterpart in the reference (its TimerOutputs sections time whole
  * algorithm steps, src/algorithms/groundstate/dmrg.jl:139-150).  When enabled,

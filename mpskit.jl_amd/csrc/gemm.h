@@ -38,11 +38,31 @@ struct GemmProb {
   double alpha, beta;
 };
 
+// Up to MAX_PROB problems passed by value as the kernel argument.
 struct GemmBatch {
   GemmProb p[MAX_PROB];
   int32_t nprob;
   int32_t total_tiles;
 };
+
+// Any number of problems in a device-resident array (problems sorted by
+// decreasing per-tile work, tile_start ascending), worked through by a
+// persistent grid: workgroup b takes tiles b, b + gridDim.x, ...
+// Pointer fields of the array may be relative: bits 62-63 name a base slot
+// (1..3) of `base` and the low bits are a byte offset into it, so one array
+// serves every call whose inputs and outputs live at other addresses.
+constexpr int NBASE = 4;
+struct GemmGroup {
+  const GemmProb* probs;
+  int32_t nprob;
+  int32_t total_tiles;
+  const char* base[NBASE];
+};
+
+constexpr uint64_t REL_MASK = (1ull << 62) - 1;
+inline const void* rel_ptr(int slot, size_t offset) {
+  return reinterpret_cast<const void*>((static_cast<uint64_t>(slot) << 62) | offset);
+}
 
 // Host-side builders -------------------------------------------------------
 
@@ -86,5 +106,17 @@ mk_status gemm_f64(const ProbSpec* probs, int nprob, GemmShape shape, hipStream_
 // accumulation in fp32, output C is float64 column-major (alpha/beta in fp64).
 mk_status gemm_bf16x3(const ProbSpec* probs, int nprob, hipStream_t stream);
 mk_status gemm_f32(const ProbSpec* probs, int nprob, hipStream_t stream);
+
+// Grouped (persistent) launches over a device descriptor array.  Tile kinds:
+//   FP64:  TK_64 (64x64), TK_128 (128x128)
+//   mixed: TK_64, TK_128, TK_256 (256x256, 8 waves)
+enum TileKind { TK_64 = 0, TK_128 = 1, TK_256 = 2 };
+inline int tile_m(int kind) { return kind == TK_64 ? 64 : kind == TK_128 ? 128 : 256; }
+inline int tile_n(int kind) { return tile_m(kind); }
+bool tile_valid(int prec, int kind);
+// Workgroups of one persistent launch: resident workgroups per CU x CUs, capped by the tiles.
+int group_grid(int prec, int kind, GemmShape sh, int total_tiles);
+mk_status gemm_f64_group(const GemmGroup& g, int kind, GemmShape sh, int grid, hipStream_t st);
+mk_status gemm_mixed_group(const GemmGroup& g, int prec, int kind, int grid, hipStream_t st);
 
 }  // namespace mk

@@ -1,0 +1,65 @@
+// Device-side tile sources shared by the GEMM kernel families: a kernel runs
+// `for (t = first(); t < total_tiles; t += stride())` and fetches the problem
+// owning tile t.  GemmBatch launches one workgroup per tile (the loop runs once,
+// tiles remapped so that each XCD gets a contiguous range); GemmGroup launches a
+// persistent grid over a device-resident problem array.
+#pragma once
+#include "gemm.h"
+
+namespace mk {
+
+constexpr int NXCD = 8;
+
+// Workgroups are dispatched round-robin over the XCDs.  Give XCD x a
+// contiguous range of tile ids so that neighbouring tiles (which share A row
+// panels / B column panels) are resident in the same L2.
+__device__ __forceinline__ int xcd_remap(int t, int total) {
+  const int q = total / NXCD, r = total % NXCD;
+  const int x = t % NXCD, local = t / NXCD;
+  return x * q + min(x, r) + local;
+}
+
+__device__ __forceinline__ int src_first(const GemmBatch& b) { return xcd_remap(blockIdx.x, b.total_tiles); }
+__device__ __forceinline__ int src_stride(const GemmBatch& b) { return b.total_tiles; }
+__device__ __forceinline__ const GemmProb& src_prob(const GemmBatch& b, int t) {
+  int pi = 0;
+  while (pi + 1 < b.nprob && t >= b.p[pi + 1].tile_start) ++pi;
+  return b.p[pi];
+}
+
+__device__ __forceinline__ int src_first(const GemmGroup&) { return blockIdx.x; }
+__device__ __forceinline__ int src_stride(const GemmGroup&) { return gridDim.x; }
+__device__ __forceinline__ const GemmProb& src_prob(const GemmGroup& g, int t) {
+  int lo = 0, hi = g.nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g.probs[mid].tile_start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return g.probs[lo];
+}
+
+// Resolve a descriptor pointer: absolute for GemmBatch, (slot, offset) for GemmGroup.
+template <class T>
+__device__ __forceinline__ T* resolve(const GemmBatch&, T* p) { return p; }
+template <class T>
+__device__ __forceinline__ T* resolve(const GemmGroup& g, T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const int slot = (int)(v >> 62);
+  if (slot == 0) return p;
+  return reinterpret_cast<T*>(const_cast<char*>(g.base[slot]) + (v & REL_MASK));
+}
+
+// Position of tile u of a problem: tile rows are grouped GROUP_M at a time so
+// that consecutive tiles reuse the same A row panels.
+constexpr int GROUP_M = 8;
+__device__ __forceinline__ void tile_coords(const GemmProb& P, int u, int& tm, int& tn) {
+  const int per_group = GROUP_M * P.tiles_n;
+  const int g = u / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsize = min(GROUP_M, P.tiles_m - first_m);
+  tm = first_m + (u % per_group) % gsize;
+  tn = (u % per_group) / gsize;
+}
+
+}  // namespace mk

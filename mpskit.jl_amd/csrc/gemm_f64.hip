@@ -3,8 +3,10 @@
 // lanes in a 2x2 arrangement.  One workgroup computes one BM x BN output tile
 // of one problem, summing every segment of that problem into the same
 // accumulators before a single fused epilogue (alpha/beta and optional CB term).
-#include "gemm.h"
+#include "gemm_dev.h"
 #include "prof.h"
+
+#include <algorithm>
 
 namespace mk {
 namespace {
@@ -15,17 +17,6 @@ typedef double dx2 __attribute__((ext_vector_type(2)));
 constexpr int NT = 256;
 constexpr int BK = 32;
 constexpr int PAD = 16;     // LDS row padding (doubles): alternates 128-B bank halves
-constexpr int NXCD = 8;
-constexpr int GROUP_M = 8;  // tile-row grouping for L2 reuse
-
-// Workgroups are dispatched round-robin over the XCDs.  Give XCD x a
-// contiguous range of tile ids so that neighbouring tiles (which share A row
-// panels / B column panels) are resident in the same L2.
-__device__ __forceinline__ int xcd_remap(int t, int total) {
-  const int q = total / NXCD, r = total % NXCD;
-  const int x = t % NXCD, local = t / NXCD;
-  return x * q + min(x, r) + local;
-}
 
 // Stage a tile whose contiguous memory direction is the LDS fast direction.
 // element (r, k) lives at p[(r0 + r) + (k0 + k) * ld]; LDS tile S[k][r].
@@ -112,8 +103,8 @@ __device__ __forceinline__ void store_kdir(double (*S)[LDR], const dx2 (&reg)[R 
   }
 }
 
-template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
+template <int BM, int BN, bool TA, bool TB, class Src>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(Src src) {
   constexpr int LDA_S = BM + PAD;
   constexpr int LDB_S = BN + PAD;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -125,19 +116,10 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
   __shared__ double As[2][BK][LDA_S];
   __shared__ double Bs[2][BK][LDB_S];
 
-  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
-  int pi = 0;
-  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
-  const GemmProb& P = batch.p[pi];
-
-  // grouped tile order inside the problem
-  const int u = t - P.tile_start;
-  const int per_group = GROUP_M * P.tiles_n;
-  const int g = u / per_group;
-  const int first_m = g * GROUP_M;
-  const int gsize = min(GROUP_M, P.tiles_m - first_m);
-  const int tm = first_m + (u % per_group) % gsize;
-  const int tn = (u % per_group) / gsize;
+  for (int t = src_first(src); t < src.total_tiles; t += src_stride(src)) {
+  const GemmProb& P = src_prob(src, t);
+  int tm, tn;
+  tile_coords(P, t - P.tile_start, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
 
   const int lane = threadIdx.x & 63;
@@ -160,8 +142,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
   auto load = [&](int s, int k0) {
     const GemmSeg& S = P.seg[s];
     const bool vec = S.vec != 0;
-    const double* A = static_cast<const double*>(S.A);
-    const double* B = static_cast<const double*>(S.B);
+    const double* A = static_cast<const double*>(resolve(src, S.A));
+    const double* B = static_cast<const double*>(resolve(src, S.B));
     if (!TA) load_fastdir<BM, LDA_S>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
     else     load_kdir<BM, LDA_S>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
     if (!TB) load_kdir<BN, LDB_S>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
@@ -214,6 +196,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
   // block is C^T, so the lane holds C[row li][col lk + 4 r]: the 16 lanes of a
   // quarter-wave store 16 consecutive rows (128 B) of one column.
   const double alpha = P.alpha, beta = P.beta;
+  double* Cp = resolve(src, P.C);
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
     const int row = m0 + wm0 + i * 16 + li;
@@ -224,13 +207,14 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch batch) {
       for (int r = 0; r < 4; ++r) {
         const int col = n0 + wn0 + j * 16 + lk + 4 * r;
         if (col >= P.N) continue;
-        double* cp = P.C + (int64_t)row + (int64_t)col * P.ldc;
+        double* cp = Cp + (int64_t)row + (int64_t)col * P.ldc;
         double v = alpha * acc[i][j][r];
         if (beta != 0.0) v += beta * *cp;
         *cp = v;
       }
     }
   }
+  }  // tiles
 }
 
 template <int BM, int BN>
@@ -270,15 +254,51 @@ mk_status launch_cfg(const ProbSpec* probs, int nprob, GemmShape sh, hipStream_t
   if (total == 0) return MK_OK;
   dim3 grid(total), block(NT);
   ProfScope prof("gemm_f64", stream);
-  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false>), grid, block, 0, stream, b);
-  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false>), grid, block, 0, stream, b);
-  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true>), grid, block, 0, stream, b);
-  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true>), grid, block, 0, stream, b);
+  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, GemmBatch>), grid, block, 0, stream, b);
+  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, GemmBatch>), grid, block, 0, stream, b);
+  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, GemmBatch>), grid, block, 0, stream, b);
+  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, GemmBatch>), grid, block, 0, stream, b);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
 }
 
+template <int BM, int BN>
+mk_status launch_group(const GemmGroup& g, GemmShape sh, int grid, hipStream_t stream) {
+  if (g.total_tiles == 0) return MK_OK;
+  dim3 gr(grid), block(NT);
+  ProfScope prof("gemm_f64", stream);
+  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, GemmGroup>), gr, block, 0, stream, g);
+  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, GemmGroup>), gr, block, 0, stream, g);
+  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, GemmGroup>), gr, block, 0, stream, g);
+  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, GemmGroup>), gr, block, 0, stream, g);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+template <int BM, int BN>
+int occupancy() {
+  static const int occ = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64_kernel<BM, BN, false, false, GemmGroup>, NT, 0) !=
+        hipSuccess)
+      n = 1;
+    return n > 0 ? n : 1;
+  }();
+  return occ;
+}
+
 }  // namespace
+
+int f64_group_grid(int kind, int total_tiles, int cus) {
+  const int occ = kind == TK_128 ? occupancy<128, 128>() : occupancy<64, 64>();
+  return (int)std::min<int64_t>(total_tiles, (int64_t)occ * cus);
+}
+
+mk_status gemm_f64_group(const GemmGroup& g, int kind, GemmShape sh, int grid, hipStream_t st) {
+  if (kind == TK_128) return launch_group<128, 128>(g, sh, grid, st);
+  MK_REQUIRE(kind == TK_64, MK_BAD_ARG, "gemm_f64_group: tile kind %d", kind);
+  return launch_group<64, 64>(g, sh, grid, st);
+}
 
 mk_status gemm_f64(const ProbSpec* probs, int nprob, GemmShape shape, hipStream_t stream) {
   MK_REQUIRE(nprob >= 0 && nprob <= MAX_PROB, MK_BAD_ARG, "gemm_f64: %d problems (max %d)", nprob, MAX_PROB);

@@ -14,10 +14,11 @@
 // by A and B.
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
-#include "gemm.h"
+#include "gemm_dev.h"
 #include "prof.h"
 
 namespace mk {
@@ -30,8 +31,6 @@ typedef unsigned int ux4 __attribute__((ext_vector_type(4)));
 constexpr int NT = 256;
 constexpr int ROWB = 64;          // bytes of K per LDS row per step
 constexpr int ROWP = ROWB + 16;   // padded LDS row (80 B: conflict-free 16-B reads)
-constexpr int NXCD = 8;
-constexpr int GROUP_M = 8;
 
 enum Mode { M_BF16X3 = 0, M_F32 = 1 };
 
@@ -46,12 +45,6 @@ int variant() {
     return (int)V_BIG;
   }();
   return v;
-}
-
-__device__ __forceinline__ int xcd_remap(int t, int total) {
-  const int q = total / NXCD, r = total % NXCD;
-  const int x = t % NXCD, local = t / NXCD;
-  return x * q + min(x, r) + local;
 }
 
 // Epilogue of an FM x FN grid of 16x16 fp32 accumulator blocks.  The MFMAs run
@@ -112,8 +105,8 @@ __device__ __forceinline__ void store_slab(char* S, const ux4 (&reg)[R * ROWB / 
   }
 }
 
-template <int BM, int BN, int MODE>
-__global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
+template <int BM, int BN, int MODE, class Src>
+__global__ __launch_bounds__(NT) void gemm_mixed_kernel(Src src) {
   constexpr int NPART = MODE == M_BF16X3 ? 2 : 1;   // hi/lo arrays per operand
   constexpr int EB = MODE == M_BF16X3 ? 2 : 4;      // element bytes
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -124,18 +117,10 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
   __shared__ __attribute__((aligned(16))) char As[2][NPART][BM * ROWP];
   __shared__ __attribute__((aligned(16))) char Bs[2][NPART][BN * ROWP];
 
-  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
-  int pi = 0;
-  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
-  const GemmProb& P = batch.p[pi];
-
-  const int u = t - P.tile_start;
-  const int per_group = GROUP_M * P.tiles_n;
-  const int g = u / per_group;
-  const int first_m = g * GROUP_M;
-  const int gsize = min(GROUP_M, P.tiles_m - first_m);
-  const int tm = first_m + (u % per_group) % gsize;
-  const int tn = (u % per_group) / gsize;
+  for (int t = src_first(src); t < src.total_tiles; t += src_stride(src)) {
+  const GemmProb& P = src_prob(src, t);
+  int tm, tn;
+  tile_coords(P, t - P.tile_start, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
 
   const int lane = threadIdx.x & 63;
@@ -160,11 +145,11 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
     const GemmSeg& S = P.seg[s];
     const int64_t ldab = S.lda * EB, ldbb = S.ldb * EB;
     const int64_t kb0 = (int64_t)k0 * EB, Kb = (int64_t)S.K * EB;
-    load_slab<BM>(static_cast<const char*>(S.A), ldab, m0, kb0, P.M, Kb, ra[0]);
-    load_slab<BN>(static_cast<const char*>(S.B), ldbb, n0, kb0, P.N, Kb, rb[0]);
+    load_slab<BM>(static_cast<const char*>(resolve(src, S.A)), ldab, m0, kb0, P.M, Kb, ra[0]);
+    load_slab<BN>(static_cast<const char*>(resolve(src, S.B)), ldbb, n0, kb0, P.N, Kb, rb[0]);
     if (NPART == 2) {
-      load_slab<BM>(static_cast<const char*>(S.Alo), ldab, m0, kb0, P.M, Kb, ra[NPART - 1]);
-      load_slab<BN>(static_cast<const char*>(S.Blo), ldbb, n0, kb0, P.N, Kb, rb[NPART - 1]);
+      load_slab<BM>(static_cast<const char*>(resolve(src, S.Alo)), ldab, m0, kb0, P.M, Kb, ra[NPART - 1]);
+      load_slab<BN>(static_cast<const char*>(resolve(src, S.Blo)), ldbb, n0, kb0, P.N, Kb, rb[NPART - 1]);
     }
   };
   auto store = [&](int buf) {
@@ -233,7 +218,8 @@ __global__ __launch_bounds__(NT) void gemm_mixed_kernel(GemmBatch batch) {
     k0 = nk0;
   }
 
-  store_tile<FM, FN>(acc, P.C, P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
+  store_tile<FM, FN>(acc, resolve(src, P.C), P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
+  }  // tiles
 }
 
 // ---------------------------------------------------------------------------
@@ -277,8 +263,8 @@ __device__ __forceinline__ void store_slab_big(char* S, const ux4 (&reg)[R * ROW
   }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
+template <int MODE, class Src>
+__global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(Src src) {
   constexpr int NPART = MODE == M_BF16X3 ? 2 : 1;
   constexpr int EB = MODE == M_BF16X3 ? 2 : 4;
   constexpr int WM = 128, WN = 64;
@@ -288,17 +274,10 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
   __shared__ __attribute__((aligned(16))) char As[2][NPART][BIG * ROWB];
   __shared__ __attribute__((aligned(16))) char Bs[2][NPART][BIG * ROWB];
 
-  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
-  int pi = 0;
-  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
-  const GemmProb& P = batch.p[pi];
-  const int u = t - P.tile_start;
-  const int per_group = GROUP_M * P.tiles_n;
-  const int g = u / per_group;
-  const int first_m = g * GROUP_M;
-  const int gsize = min(GROUP_M, P.tiles_m - first_m);
-  const int tm = first_m + (u % per_group) % gsize;
-  const int tn = (u % per_group) / gsize;
+  for (int t = src_first(src); t < src.total_tiles; t += src_stride(src)) {
+  const GemmProb& P = src_prob(src, t);
+  int tm, tn;
+  tile_coords(P, t - P.tile_start, tm, tn);
   const int m0 = tm * BIG, n0 = tn * BIG;
 
   const int lane = threadIdx.x & 63;
@@ -323,11 +302,11 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
     const GemmSeg& S = P.seg[s];
     const int64_t ldab = S.lda * EB, ldbb = S.ldb * EB;
     const int64_t kb0 = (int64_t)k0 * EB, Kb = (int64_t)S.K * EB;
-    load_slab_big<BIG>(static_cast<const char*>(S.A), ldab, m0, kb0, P.M, Kb, ra[0]);
-    load_slab_big<BIG>(static_cast<const char*>(S.B), ldbb, n0, kb0, P.N, Kb, rb[0]);
+    load_slab_big<BIG>(static_cast<const char*>(resolve(src, S.A)), ldab, m0, kb0, P.M, Kb, ra[0]);
+    load_slab_big<BIG>(static_cast<const char*>(resolve(src, S.B)), ldbb, n0, kb0, P.N, Kb, rb[0]);
     if (NPART == 2) {
-      load_slab_big<BIG>(static_cast<const char*>(S.Alo), ldab, m0, kb0, P.M, Kb, ra[NPART - 1]);
-      load_slab_big<BIG>(static_cast<const char*>(S.Blo), ldbb, n0, kb0, P.N, Kb, rb[NPART - 1]);
+      load_slab_big<BIG>(static_cast<const char*>(resolve(src, S.Alo)), ldab, m0, kb0, P.M, Kb, ra[NPART - 1]);
+      load_slab_big<BIG>(static_cast<const char*>(resolve(src, S.Blo)), ldbb, n0, kb0, P.N, Kb, rb[NPART - 1]);
     }
   };
   auto store = [&](int buf) {
@@ -394,7 +373,8 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(GemmBatch batch) {
     k0 = nk0;
   }
 
-  store_tile<FM, FN>(acc, P.C, P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
+  store_tile<FM, FN>(acc, resolve(src, P.C), P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
+  }  // tiles
 }
 
 // ---------------------------------------------------------------------------
@@ -470,17 +450,10 @@ __global__ __launch_bounds__(PNT) void gemm_mixed_pipe_kernel(GemmBatch batch) {
 
   __shared__ __attribute__((aligned(16))) char smem[NSTAGE * SBYTES];
 
-  const int t = xcd_remap(blockIdx.x, batch.total_tiles);
-  int pi = 0;
-  while (pi + 1 < batch.nprob && t >= batch.p[pi + 1].tile_start) ++pi;
-  const GemmProb& P = batch.p[pi];
-  const int u = t - P.tile_start;
-  const int per_group = GROUP_M * P.tiles_n;
-  const int g = u / per_group;
-  const int first_m = g * GROUP_M;
-  const int gsize = min(GROUP_M, P.tiles_m - first_m);
-  const int tm = first_m + (u % per_group) % gsize;
-  const int tn = (u % per_group) / gsize;
+  const int t = src_first(batch);
+  const GemmProb& P = src_prob(batch, t);
+  int tm, tn;
+  tile_coords(P, t - P.tile_start, tm, tn);
   const int m0 = tm * PBM, n0 = tn * PBN;
 
   const int lane = threadIdx.x & 63;
@@ -612,9 +585,9 @@ mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
   if constexpr (VAR == V_PIPE)
     hipLaunchKernelGGL((gemm_mixed_pipe_kernel<MODE>), dim3(total), dim3(PNT), 0, stream, b);
   else if constexpr (VAR == V_BIG)
-    hipLaunchKernelGGL((gemm_mixed_big_kernel<MODE>), dim3(total), dim3(BNT), 0, stream, b);
+    hipLaunchKernelGGL((gemm_mixed_big_kernel<MODE, GemmBatch>), dim3(total), dim3(BNT), 0, stream, b);
   else
-    hipLaunchKernelGGL((gemm_mixed_kernel<BM, BN, MODE>), dim3(total), dim3(NT), 0, stream, b);
+    hipLaunchKernelGGL((gemm_mixed_kernel<BM, BN, MODE, GemmBatch>), dim3(total), dim3(NT), 0, stream, b);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
 }
@@ -645,7 +618,43 @@ mk_status dispatch(const ProbSpec* probs, int nprob, hipStream_t stream) {
   return launch_mixed<64, 64, MODE, V_TILE>(probs, nprob, stream);
 }
 
+template <int MODE>
+mk_status launch_group(const GemmGroup& g, int kind, int grid, hipStream_t stream) {
+  if (g.total_tiles == 0) return MK_OK;
+  ProfScope prof(MODE == M_BF16X3 ? "gemm_bf16x3" : "gemm_f32", stream);
+  if (kind == TK_256)
+    hipLaunchKernelGGL((gemm_mixed_big_kernel<MODE, GemmGroup>), dim3(grid), dim3(BNT), 0, stream, g);
+  else if (kind == TK_128)
+    hipLaunchKernelGGL((gemm_mixed_kernel<128, 128, MODE, GemmGroup>), dim3(grid), dim3(NT), 0, stream, g);
+  else
+    hipLaunchKernelGGL((gemm_mixed_kernel<64, 64, MODE, GemmGroup>), dim3(grid), dim3(NT), 0, stream, g);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+template <class K>
+int occ_of(K kern, int threads) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, threads, 0) != hipSuccess) n = 1;
+  return n > 0 ? n : 1;
+}
+
 }  // namespace
+
+int mixed_group_grid(int prec, int kind, int total_tiles, int cus) {
+  // occupancy is a property of the kernel's resources; the BF16X3 instances are the larger
+  static const int occ[3] = {occ_of(gemm_mixed_kernel<64, 64, M_BF16X3, GemmGroup>, NT),
+                             occ_of(gemm_mixed_kernel<128, 128, M_BF16X3, GemmGroup>, NT),
+                             occ_of(gemm_mixed_big_kernel<M_BF16X3, GemmGroup>, BNT)};
+  (void)prec;
+  return (int)std::min<int64_t>(total_tiles, (int64_t)occ[kind] * cus);
+}
+
+mk_status gemm_mixed_group(const GemmGroup& g, int prec, int kind, int grid, hipStream_t st) {
+  MK_REQUIRE(kind >= TK_64 && kind <= TK_256, MK_BAD_ARG, "gemm_mixed_group: tile kind %d", kind);
+  if (prec == MK_BF16X3) return launch_group<M_BF16X3>(g, kind, grid, st);
+  return launch_group<M_F32>(g, kind, grid, st);
+}
 
 mk_status gemm_bf16x3(const ProbSpec* probs, int nprob, hipStream_t stream) {
   return dispatch<M_BF16X3>(probs, nprob, stream);

@@ -109,6 +109,10 @@ struct Dev {
 
 }  // namespace
 
+void lanczos_lowest(const std::vector<double>& G, int ld, int k, double& lam, std::vector<double>& s) {
+  lowest(G, ld, k, lam, s);
+}
+
 // inner Krylov stopping factor for the mixed modes: stop the low-precision
 // cycle once its estimated residual is below eta * (float64 residual at the
 // start of the cycle), i.e. aim for a few digits of contraction per cycle.

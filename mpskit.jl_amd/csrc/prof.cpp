@@ -40,6 +40,9 @@ ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
     on = g_on;
   }
   if (!on) return;
+  // launches being captured into a graph are not timed
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
   if (hipEventCreate(&start) != hipSuccess) {
     start = nullptr;
     return;

@@ -8,7 +8,7 @@ call that needs the library loads it and raises if it is missing.
 from . import models, tensors
 from ._lib import BF16X3, FP32, FP64, EXPORTED, LIB_PATH, MKError, NotConvergedError, SpaceMismatch, load
 from .derivatives import (
-    DerivativeOperator, JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, MPO_AC2_Hamiltonian, gemm,
+    BatchedDerivative, autotune_stats, DerivativeOperator, JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, MPO_AC2_Hamiltonian, gemm,
     MPO_AC_Hamiltonian, MPO_C_Hamiltonian, MPODerivativeOperator, PreparedDerivative, prepare_operator,
 )
 from .transfer import (
@@ -18,7 +18,7 @@ from .transfer import (
 
 __all__ = [
     "models", "tensors", "BF16X3", "FP32", "FP64", "EXPORTED", "LIB_PATH", "MKError", "NotConvergedError",
-    "SpaceMismatch", "load", "gemm", "DerivativeOperator", "JordanMPO_AC2_Hamiltonian", "JordanMPO_AC_Hamiltonian",
+    "SpaceMismatch", "load", "gemm", "BatchedDerivative", "autotune_stats", "DerivativeOperator", "JordanMPO_AC2_Hamiltonian", "JordanMPO_AC_Hamiltonian",
     "MPO_AC2_Hamiltonian", "MPO_AC_Hamiltonian", "MPO_C_Hamiltonian", "MPODerivativeOperator",
     "PreparedDerivative", "prepare_operator", "ProductTransferMatrix", "RegTransferMatrix",
     "SingleTransferMatrix", "TransferMatrix", "regularize", "regularize_", "transfer_left", "transfer_right",

@@ -66,6 +66,13 @@ _SIGS = {
     "mk_eigsolve": ([_P, _P, _INT, _INT, _D, _INT, C.POINTER(_D), C.POINTER(_D), C.POINTER(_INT),
                      C.POINTER(_INT)], _INT),
     "mk_gemm": ([_P, _INT, _INT, _I64, _I64, _I64, _D, _P, _I64, _P, _I64, _D, _P, _I64, _INT], _INT),
+    "mk_batch_create": ([_P, _INT, C.POINTER(_P), C.POINTER(_P)], _INT),
+    "mk_batch_layout": ([_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_INT)], _INT),
+    "mk_batch_apply": ([_P, _P, _P, _INT], _INT),
+    "mk_batch_eigsolve": ([_P, _P, _INT, _INT, _D, _INT, _INT, C.POINTER(_D), C.POINTER(_D), C.POINTER(_INT),
+                           C.POINTER(_INT)], _INT),
+    "mk_batch_free": ([_P], _INT),
+    "mk_autotune_stats": ([C.POINTER(_INT), C.POINTER(_INT)], _INT),
     "mk_prof_enable": ([_INT], _INT),
     "mk_prof_reset": ([], _INT),
     "mk_prof_query": ([C.c_char_p, C.POINTER(_D), C.POINTER(_I64)], _INT),

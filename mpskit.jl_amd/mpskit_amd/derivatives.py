@@ -121,6 +121,98 @@ class PreparedDerivative(DerivativeOperator):
             pass
 
 
+class BatchedDerivative:
+    """Many prepared square operators applied and solved together (``mk_batch_*``).
+
+    The reference applies one operator per call (derivatives.jl:8-11) and runs
+    one local eigensolve per ``fixedpoint`` (fixedpoint.jl:15-18); a batch runs
+    the applies of all instances as one grouped launch and their Lanczos solves
+    in lockstep.  Vectors are packed into one segmented device buffer
+    (``pack``/``unpack``); the operators must stay alive while the batch does.
+    """
+
+    def __init__(self, ops):
+        self.ops = list(ops)
+        if not self.ops:
+            raise ValueError("BatchedDerivative: no operators")
+        for op in self.ops:
+            if op.x_shape != op.y_shape:
+                raise SpaceMismatch(1, "BatchedDerivative: operators must be square")
+        self.ctx = self.ops[0].ctx
+        n = len(self.ops)
+        arr = (C.c_void_p * n)(*[op.handle for op in self.ops])
+        h = C.c_void_p()
+        check(load().mk_batch_create(self.ctx.handle, n, arr, C.byref(h)))
+        self.handle = h
+        offs = (C.c_int64 * n)()
+        tot, grp = C.c_int64(), C.c_int()
+        check(load().mk_batch_layout(h, offs, C.byref(tot), C.byref(grp)))
+        self.offsets = [int(o) for o in offs]
+        self.total = int(tot.value)
+        self.grouped = bool(grp.value)
+
+    def pack(self, xs) -> torch.Tensor:
+        flat = torch.zeros(self.total, dtype=torch.float64, device=xs[0].device)
+        for op, o, x in zip(self.ops, self.offsets, xs):
+            x = _req(x, op.x_shape, "x")
+            n = x.numel()
+            flat[o:o + n].copy_(x.permute(*reversed(range(x.dim()))).reshape(-1))
+        return flat
+
+    def unpack(self, flat: torch.Tensor):
+        out = []
+        for op, o in zip(self.ops, self.offsets):
+            n = 1
+            for d in op.x_shape:
+                n *= d
+            out.append(T.as_colmajor(flat[o:o + n].reshape(tuple(reversed(op.x_shape)))
+                                     .permute(*reversed(range(len(op.x_shape))))))
+        return out
+
+    def apply(self, flat: torch.Tensor, precision: PrecisionArg = "fp64") -> torch.Tensor:
+        """y = H x for every instance of a packed vector."""
+        if flat.numel() != self.total or flat.dtype != torch.float64 or not flat.is_contiguous():
+            raise SpaceMismatch(1, "BatchedDerivative.apply: wrong packed vector")
+        y = torch.zeros_like(flat)
+        self.ctx.sync_stream()
+        check(load().mk_batch_apply(self.handle, T.ptr(flat), T.ptr(y), _prec(precision)))
+        return y
+
+    def eigsolve(self, xs, krylovdim: int = 30, maxiter: int = 100, tol: float = 1e-12,
+                 precision: PrecisionArg = "fp64", use_graph: bool = True):
+        """Lowest eigenpairs of all instances; returns (eigenvalues, eigenvectors, infos)."""
+        flat = self.pack(xs)
+        n = len(self.ops)
+        ev, rn = (C.c_double * n)(), (C.c_double * n)()
+        nops, conv = (C.c_int * n)(), (C.c_int * n)()
+        self.ctx.sync_stream()
+        check(load().mk_batch_eigsolve(self.handle, T.ptr(flat), int(krylovdim), int(maxiter), float(tol),
+                                       _prec(precision), int(bool(use_graph)), ev, rn, nops, conv))
+        infos = [dict(normres=rn[i], numops=nops[i], converged=bool(conv[i])) for i in range(n)]
+        return [ev[i] for i in range(n)], self.unpack(flat), infos
+
+    def free(self) -> None:
+        h = getattr(self, "handle", None)
+        if sys.is_finalizing():
+            return
+        if h is not None and h.value:
+            load().mk_batch_free(h)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def autotune_stats():
+    """(decisions, cache hits) of the grouped-GEMM tile autotuner in this process."""
+    d, h = C.c_int(), C.c_int()
+    check(load().mk_autotune_stats(C.byref(d), C.byref(h)))
+    return d.value, h.value
+
+
 # ---------------------------------------------------------------------------
 # Generic MPO derivatives (mpo_derivatives.jl)
 # ---------------------------------------------------------------------------
