@@ -235,6 +235,48 @@ mk_status mk_batch_free(mk_batch b);
  * counterpart in the reference). */
 mk_status mk_autotune_stats(int* decisions, int* hits);
 
+/* ---- learned Davidson preconditioner ------------------------------------------
+ * No counterpart in the reference (its local solver is KrylovKit's Lanczos,
+ * fixedpoint.jl:15-18): a Davidson solver for the same lowest-eigenpair
+ * problem whose correction vectors come from a small network, for callers that
+ * solve many similar local problems (sweeps, parameter scans).
+ *
+ * The network is a 3-layer FP32 MLP (512 -> 256 -> 256 -> 256, tanh) acting on
+ * 256-element blocks of the residual: input = the residual block and the
+ * diagonally preconditioned residual block q = r / (theta - diag H), output y =
+ * a multiplicative refinement of the diagonal correction, t = q (1 + y); the
+ * untrained network (zero last layer) is the diagonal correction.  Training
+ * minimises the mean over blocks of |t*| (1 - cos(q (1 + y), t*)) with t* the
+ * ideal correction.  Parameters are flattened [W0 b0 W1 b1 W2 b2] with
+ * row-major (out x in) weights (mk_mlp_param_count floats).  Training pairs
+ * (features, ideal correction of a Davidson iterate) are collected on the
+ * device by mk_davidson(collect = 1); mk_mlp_train runs Adam on random
+ * minibatches of them with the library's forward/backward kernels. */
+typedef struct mk_mlp_s* mk_mlp;
+mk_status mk_mlp_create(mk_ctx ctx, uint64_t seed, int64_t capacity, mk_mlp* out);
+mk_status mk_mlp_free(mk_mlp m);
+mk_status mk_mlp_param_count(int64_t* n);
+mk_status mk_mlp_get_params(mk_mlp m, float* host);
+mk_status mk_mlp_set_params(mk_mlp m, const float* host);
+/* Y (rows x 256) = MLP(X (rows x 512)), device pointers. */
+mk_status mk_mlp_forward(mk_mlp m, const float* X, int64_t rows, float* Y);
+/* Loss to the targets T (rows x 256) and its parameter gradient (host, optional). */
+mk_status mk_mlp_gradient(mk_mlp m, const float* X, const float* T, int64_t rows, float* grads_host, float* loss);
+mk_status mk_mlp_dataset_size(mk_mlp m, int64_t* rows);
+/* Adam steps on random minibatches of training rows [row_begin, row_end)
+ * (row_end < 0: all rows); minibatch losses of the first and last step. */
+mk_status mk_mlp_train(mk_mlp m, int steps, int batch, float lr, uint64_t seed, int64_t row_begin, int64_t row_end,
+                       float* first_loss, float* last_loss);
+/* Loss of the current parameters on training rows [row_begin, row_end). */
+mk_status mk_mlp_eval(mk_mlp m, int64_t row_begin, int64_t row_end, float* loss);
+/* Lowest eigenpair by Davidson iteration with float64 Rayleigh-Ritz and
+ * residuals.  mode 0: residual correction, 1: diagonal correction, 2: MLP
+ * correction (mlp required).  Falls back to mk_eigsolve (Lanczos) from the
+ * current vector if not converged after maxiter steps.  collect = 1 appends
+ * training pairs of the iterates to mlp's training set. */
+mk_status mk_davidson(mk_op op, double* x, int maxdim, int maxiter, double tol, mk_mlp mlp, int mode, int collect,
+                      double* eigval, double* resnorm, int* numops, int* converged);
+
 /* ---- kernel timing ---------------------------------------------------------
  * No counterpart in the reference (its TimerOutputs sections time whole
  * algorithm steps, src/algorithms/groundstate/dmrg.jl:139-150).  When enabled,

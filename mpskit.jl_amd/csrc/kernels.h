@@ -53,6 +53,8 @@ mk_status k_tr_mid(const double* t1, const double* O, int64_t Da, int wl, int d,
 mk_status k_phys2(const double* P, const double* x, int64_t Dl, int d1, int64_t Dr, int d2,
                   double alpha, double beta, double* y, hipStream_t st);
 // physical one-site term: y[a,b,c] = alpha*sum_h P[b,h] x[a,h,c] + beta*y
+mk_status k_diag_nn(const double* IC, const double* BE, const double* P, int64_t Dl, int64_t Dr, int d1, int d2,
+                    double* out, hipStream_t st);
 mk_status k_phys1(const double* P, const double* x, int64_t Dl, int d, int64_t Dr,
                   double alpha, double beta, double* y, hipStream_t st);
 

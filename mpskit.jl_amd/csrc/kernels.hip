@@ -256,6 +256,26 @@ __global__ void phys2_reg_kernel(const double* __restrict__ P, const double* __r
   }
 }
 
+// Diagonal of the nearest-neighbour two-site Jordan operator x*IC + BE*x + CB(x):
+// diag[row + M col] = IC[col, col] + BE[row, row] + CB[b, e, b, e], row = a + Dl b,
+// col = c + Dr e (absent terms are null).
+__global__ void diag_nn_kernel(const double* __restrict__ IC, const double* __restrict__ BE,
+                               const double* __restrict__ P, int64_t Dl, int64_t Dr, int d1, int d2,
+                               double* __restrict__ out) {
+  const int64_t M = Dl * d1, N = Dr * d2;
+  GRID_STRIDE(idx, M * N) {
+    const int64_t row = idx % M, col = idx / M;
+    double v = 0.0;
+    if (IC) v += IC[col + N * col];
+    if (BE) v += BE[row + M * row];
+    if (P) {
+      const int b = (int)(row / Dl), e = (int)(col / Dr);
+      v += P[b + d1 * (e + d2 * (b + d1 * e))];
+    }
+    out[idx] = v;
+  }
+}
+
 __global__ void phys1_kernel(const double* __restrict__ P, const double* __restrict__ x, int64_t Dl,
                              int d, int64_t Dr, double alpha, double beta, double* __restrict__ y) {
   const int64_t n = Dl * d * Dr;
@@ -516,6 +536,10 @@ mk_status k_phys2(const double* P, const double* x, int64_t Dl, int d1, int64_t 
     case 16: LAUNCH_1D(phys2_reg_kernel<16>, Dl * Dr, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
     default: LAUNCH_1D(phys2_kernel, Dl * d1 * Dr * d2, st, P, x, Dl, d1, Dr, d2, alpha, beta, y);
   }
+}
+mk_status k_diag_nn(const double* IC, const double* BE, const double* P, int64_t Dl, int64_t Dr, int d1, int d2,
+                    double* out, hipStream_t st) {
+  LAUNCH_1D(diag_nn_kernel, Dl * d1 * Dr * d2, st, IC, BE, P, Dl, Dr, d1, d2, out);
 }
 mk_status k_phys1(const double* P, const double* x, int64_t Dl, int d, int64_t Dr, double alpha,
                   double beta, double* y, hipStream_t st) {

@@ -321,6 +321,20 @@ mk_status op_apply(Op* op, const double* x, double* y, double alpha, double beta
   return MK_NOT_SUPPORTED;
 }
 
+// Diagonal of a square operator as far as the Davidson preconditioner needs it:
+// the IC, BE and two-site physical terms of Jordan two-site operators (exact for
+// nearest-neighbour chains), zero for every other term and operator kind.
+mk_status op_diagonal(Op* op, double* diag, hipStream_t st) {
+  const int64_t n = op->x_elems;
+  if (op->kind != OP_AC2_JOR) {
+    MK_CHECK_HIP(hipMemsetAsync(diag, 0, (size_t)n * 8, st));
+    return MK_OK;
+  }
+  return k_diag_nn(op->IC.present() ? op->IC.f64() : nullptr, op->BE.present() ? op->BE.f64() : nullptr,
+                   op->hasP ? static_cast<const double*>(op->P.ptr) : nullptr, op->Dl, op->Dr, (int)op->d1,
+                   (int)op->d2, diag, st);
+}
+
 }  // namespace mk
 
 using namespace mk;

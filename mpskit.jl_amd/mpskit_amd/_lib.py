@@ -73,6 +73,19 @@ _SIGS = {
                            C.POINTER(_INT)], _INT),
     "mk_batch_free": ([_P], _INT),
     "mk_autotune_stats": ([C.POINTER(_INT), C.POINTER(_INT)], _INT),
+    "mk_mlp_create": ([_P, C.c_uint64, _I64, C.POINTER(_P)], _INT),
+    "mk_mlp_free": ([_P], _INT),
+    "mk_mlp_param_count": ([C.POINTER(_I64)], _INT),
+    "mk_mlp_get_params": ([_P, _P], _INT),
+    "mk_mlp_set_params": ([_P, _P], _INT),
+    "mk_mlp_forward": ([_P, _P, _I64, _P], _INT),
+    "mk_mlp_gradient": ([_P, _P, _P, _I64, _P, C.POINTER(C.c_float)], _INT),
+    "mk_mlp_dataset_size": ([_P, C.POINTER(_I64)], _INT),
+    "mk_mlp_train": ([_P, _INT, _INT, C.c_float, C.c_uint64, _I64, _I64, C.POINTER(C.c_float),
+                      C.POINTER(C.c_float)], _INT),
+    "mk_mlp_eval": ([_P, _I64, _I64, C.POINTER(C.c_float)], _INT),
+    "mk_davidson": ([_P, _P, _INT, _INT, _D, _P, _INT, _INT, C.POINTER(_D), C.POINTER(_D), C.POINTER(_INT),
+                     C.POINTER(_INT)], _INT),
     "mk_prof_enable": ([_INT], _INT),
     "mk_prof_reset": ([], _INT),
     "mk_prof_query": ([C.c_char_p, C.POINTER(_D), C.POINTER(_I64)], _INT),

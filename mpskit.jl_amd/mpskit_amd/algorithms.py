@@ -64,6 +64,28 @@ class Lanczos:
 
 
 @dataclass
+class Davidson:
+    """Local solver by Davidson iteration (``mk_davidson``) instead of Lanczos.
+
+    ``mode``: "none" (residual corrections), "diag" (diagonal Davidson
+    corrections) or "mlp" (corrections from ``mlp``, a trained
+    ``MLPPreconditioner``); ``collect`` appends (features, ideal correction)
+    pairs of every solve's iterates to ``mlp``'s training set.  Convergence is
+    the float64 residual; non-converged solves continue with Lanczos.
+    """
+    tol: float = 1e-10
+    maxdim: int = 20
+    maxiter: int = 100
+    mode: str = "none"
+    mlp: object = None
+    collect: bool = False
+    stats: list = field(default_factory=list)   # (numops, converged) per solve
+
+    def adapt_solver(self, **kw):
+        return self.tol, self.maxdim, self.maxiter
+
+
+@dataclass
 class DMRG2:
     """Two-site DMRG (dmrg.jl:171-209).  ``trunc_dim`` is the maximal bond
     dimension kept by the truncated SVD (``truncrank``) and ``trunc_err`` an
@@ -141,7 +163,15 @@ def local_update(pos, direction, psi, H, alg: DMRG2, envs, eps_global, eps_trunc
     tol, krylovdim, maxiter = alg.alg_eigsolve.adapt_solver(decay_rate=decay_rate, g_local=eps_local,
                                                             g_global=eps_global, eps_trunc=eps_trunc)
     krylovdim = max(2, min(krylovdim, ac2.numel()))
-    _, newac2, info = Heff.eigsolve(ac2, krylovdim=krylovdim, maxiter=maxiter, tol=tol, precision=alg.precision)
+    es = alg.alg_eigsolve
+    if isinstance(es, Davidson):
+        from .learned import davidson
+        _, newac2, info = davidson(Heff, ac2, maxdim=krylovdim, maxiter=maxiter, tol=tol, mlp=es.mlp, mode=es.mode,
+                                   collect=es.collect)
+        es.stats.append((info["numops"], info["converged"]))
+    else:
+        _, newac2, info = Heff.eigsolve(ac2, krylovdim=krylovdim, maxiter=maxiter, tol=tol,
+                                        precision=alg.precision)
     stats["numops"] += info["numops"] + 1
     eps_trunc = gauge2(psi, pos, direction, newac2, alg.trunc_dim, alg.trunc_err)
     numops = max(1, info["numops"])

@@ -1,0 +1,134 @@
+"""Learned Davidson preconditioner on the HIP path.
+
+- forward / loss / parameter gradient of the hand-written MLP kernels against
+  the float64 oracle (oracle/mlp.py): relative error <= 2e-3 (FP32 kernels);
+- every Davidson mode converges to the Lanczos eigenvalue (1e-10 relative,
+  float64 residual), and a solve cut short by maxiter falls back to Lanczos;
+- the gain: trained on (features, ideal correction) pairs of two TFIM
+  couplings' first two sweeps, the parameters selected by the float64-residual
+  convergence count on held-out local problems of a third coupling (the
+  untrained, diagonal-correction parameters being a candidate), and used on the
+  first two sweeps of a fourth coupling, the learned Davidson solver needs
+  fewer applies than plain Lanczos (its fallback) and than Davidson without a
+  preconditioner, and no more than 10% above the diagonal correction it starts
+  from.  The applies of all four are printed.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import mlp as O  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def mk():
+    import torch
+    assert torch.cuda.is_available()
+    import mpskit_amd
+    mpskit_amd.load()
+    return mpskit_amd
+
+
+def test_mlp_forward_and_gradient_vs_oracle(mk):
+    import torch
+    rng = np.random.default_rng(3)
+    net = mk.MLPPreconditioner(seed=0, capacity=0)
+    p = (rng.standard_normal(net.get_params().size) * 0.05).astype(np.float32)
+    net.set_params(p)
+    rows = 300   # not a multiple of the 64-row tiles
+    X = rng.standard_normal((rows, 512)).astype(np.float32)
+    T = rng.standard_normal((rows, 256)).astype(np.float32)
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    Y = net.forward(Xd).cpu().numpy()
+    Yo, _ = O.forward(p.astype(np.float64), X)
+    assert np.linalg.norm(Y - Yo) / np.linalg.norm(Yo) < 2e-3
+    loss, g = net.gradient(Xd, Td)
+    lo, go = O.loss_and_grad(p.astype(np.float64), X, T)
+    assert abs(loss - lo) / abs(lo) < 2e-3
+    starts = np.cumsum([0] + [a * b + b for a, b in O.SIZES])
+    for k in range(3):
+        s = slice(starts[k], starts[k + 1])
+        e = np.linalg.norm(g[s] - go[s]) / np.linalg.norm(go[s])
+        assert e < 2e-3, (k, e)
+    net.free()
+
+
+def bond_problem(mk, L, D, g, seed, pos):
+    H = mk.FiniteMPOHamiltonian(mk.models.tfim(L, g))
+    psi = mk.FiniteMPS.random(L, 2, D, seed=seed)
+    envs = mk.FiniteEnvironments(psi, H)
+    psi.move_center(pos)
+    GL, GR = envs.leftenv(pos, psi), envs.rightenv(pos + 1, psi)
+    return mk.prepare_operator(mk.JordanMPO_AC2_Hamiltonian(GL, H[pos], H[pos + 1], GR)), psi.AC2(pos)
+
+
+def test_davidson_modes_match_lanczos_and_fallback(mk):
+    op, x0 = bond_problem(mk, 16, 32, 1.0, 4, 7)
+    e0, _, _ = op.eigsolve(x0, krylovdim=20, maxiter=200, tol=1e-11)
+    net = mk.MLPPreconditioner(seed=0, capacity=0)
+    for mode in ("none", "diag", "mlp"):
+        e, v, info = mk.davidson(op, x0, maxdim=20, maxiter=300, tol=1e-10, mlp=net, mode=mode)
+        print("davidson", mode, info, flush=True)
+        assert info["converged"] and abs(e - e0) <= 1e-10 * abs(e0)
+    # two Davidson steps cannot reach 1e-12: the solve continues with Lanczos
+    e, v, info = mk.davidson(op, x0, maxdim=20, maxiter=2, tol=1e-12, mode="none")
+    assert info["converged"] and info["numops"] > 3 and abs(e - e0) <= 1e-10 * abs(e0)
+    net.free()
+
+
+def sweep_solves(mk, L, D, g, seed, solvers, nsweeps=2):
+    """Two DMRG2 sweeps from a random state; at every bond each solver solves the
+    same local problem from the same start vector; the sweep continues with the
+    Lanczos solution.  Returns total applies per solver."""
+    from mpskit_amd.algorithms import gauge2
+    H = mk.FiniteMPOHamiltonian(mk.models.tfim(L, g))
+    psi = mk.FiniteMPS.random(L, 2, D, seed=seed)
+    envs = mk.FiniteEnvironments(psi, H)
+    tot = {k: 0 for k in solvers}
+    bonds = [(p, "right") for p in range(L - 1)] + [(p, "left") for p in range(L - 3, -1, -1)]
+    for _ in range(nsweeps):
+        for pos, direction in bonds:
+            psi.move_center(pos if direction == "right" else pos + 1)
+            GL, GR = envs.leftenv(pos, psi), envs.rightenv(pos + 1, psi)
+            op = mk.prepare_operator(mk.JordanMPO_AC2_Hamiltonian(GL, H[pos], H[pos + 1], GR))
+            x0 = psi.AC2(pos)
+            e0, v0, i0 = op.eigsolve(x0, krylovdim=20, maxiter=300, tol=1e-10)
+            tot["lanczos"] += i0["numops"]
+            for name, fn in solvers.items():
+                if name == "lanczos":
+                    continue
+                e, _, info = fn(op, x0)
+                assert info["converged"] and abs(e - e0) <= 1e-9 * abs(e0), (name, pos)
+                tot[name] += info["numops"]
+            gauge2(psi, pos, direction, v0, D, 0.0)
+            op.free()
+    return tot
+
+
+def test_learned_preconditioner_gain(mk):
+    L, D = 20, 48
+    net = mk.MLPPreconditioner(seed=1, capacity=300_000)
+    for g in (0.7, 1.3):
+        H = mk.FiniteMPOHamiltonian(mk.models.tfim(L, g))
+        psi = mk.FiniteMPS.random(L, 2, D, seed=11)
+        dav = mk.Davidson(tol=1e-10, maxdim=20, maxiter=300, mode="none", mlp=net, collect=True)
+        mk.find_groundstate(psi, H, mk.DMRG2(tol=1e-12, maxiter=2, trunc_dim=D, alg_eigsolve=dav,
+                                             exact_sweeps=True, verbosity=0))
+    assert net.dataset_size > 1000
+    held = [bond_problem(mk, L, D, 0.9, 21 + k, pos) for k, pos in enumerate((4, 8, 12, 15))]
+    sel = net.select(held, rounds=4, steps=200, lr=1e-4)
+    print("selection (applies on held-out problems)", sel, flush=True)
+    assert sel["best"] <= sel["initial"]
+    solvers = {
+        "lanczos": None,
+        "none": lambda op, x: mk.davidson(op, x, maxdim=20, maxiter=300, tol=1e-10, mode="none"),
+        "diag": lambda op, x: mk.davidson(op, x, maxdim=20, maxiter=300, tol=1e-10, mode="diag"),
+        "mlp": lambda op, x: mk.davidson(op, x, maxdim=20, maxiter=300, tol=1e-10, mlp=net, mode="mlp"),
+    }
+    tot = sweep_solves(mk, L, D, 1.0, 5, solvers)
+    print("applies over two held-out sweeps:", tot, flush=True)
+    assert tot["mlp"] < 0.8 * tot["lanczos"]
+    assert tot["mlp"] < tot["none"]
+    assert tot["mlp"] <= 1.1 * tot["diag"]
+    net.free()
