@@ -23,11 +23,13 @@ __all__ = [
     "PreparedDerivative", "prepare_operator", "ProductTransferMatrix", "RegTransferMatrix",
     "SingleTransferMatrix", "TransferMatrix", "regularize", "regularize_", "transfer_left", "transfer_right",
 ]
-from .algorithms import DMRG2, AdaptiveKrylov, Davidson, Lanczos, expectation_value, find_groundstate
+from .algorithms import (
+    DMRG2, AdaptiveKrylov, Davidson, Lanczos, expectation_value, find_groundstate, find_groundstate_batched,
+)
 from .learned import MLPPreconditioner, davidson
 from .environments import FiniteEnvironments, environments
 from .states import FiniteMPOHamiltonian, FiniteMPS
 
-__all__ += ["Davidson", "MLPPreconditioner", "davidson"]
+__all__ += ["Davidson", "MLPPreconditioner", "davidson", "find_groundstate_batched"]
 __all__ += ["DMRG2", "AdaptiveKrylov", "Lanczos", "expectation_value", "find_groundstate", "FiniteEnvironments",
             "environments", "FiniteMPOHamiltonian", "FiniteMPS"]

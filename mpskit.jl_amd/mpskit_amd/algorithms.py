@@ -99,6 +99,7 @@ class DMRG2:
     trunc_err: float = 0.0
     precision: str = "fp64"
     exact_sweeps: bool = False   # run exactly `maxiter` sweeps (no early stop)
+    gauge: str = "auto"          # truncated split: "svd", "eigh" or "auto" (see truncated_split)
 
 
 def project_complement(Y, X):
@@ -108,32 +109,65 @@ def project_complement(Y, X):
     return Ym - Xm @ (Xm.t() @ Ym)
 
 
-def gauge2(psi: FiniteMPS, pos, direction, ac2, trunc_dim, trunc_err):
-    """Truncated SVD split of the two-site center (``gauge2!``, orthoview.jl:401-420)
+def truncated_split(M, direction, trunc_dim, trunc_err, method="auto"):
+    """Rank-k split M ~ A B of a float64 matrix for the two-site gauge step.
+
+    direction "right": A is a left isometry (the new AL), B the new center;
+    direction "left":  B is a right isometry (the new AR), A the new center.
+    Returns (A, B, s, eps) with s the kept singular values and eps the 2-norm of
+    the discarded ones.  method "svd" uses the thin SVD (MatrixAlgebraKit's
+    svd_trunc in the reference, orthoview.jl:401-420); "eigh" takes the
+    isometry from the eigenvectors of the Gram matrix on the isometry's side
+    (M M^T for "right", M^T M for "left"; eigh returns orthonormal vectors to
+    working precision) and the center as the exact product with it, so no
+    singular value is ever divided by; "auto" uses eigh from 512 rows on, where
+    the GPU SVD is tens of times slower than the symmetric eigensolver.
+    """
+    m, n = M.shape
+    if method == "auto":
+        method = "eigh" if min(m, n) >= 512 else "svd"
+    if method == "svd":
+        U, S, Vh = torch.linalg.svd(M, full_matrices=False)
+        lam = S * S
+    else:
+        G = M @ M.t() if direction == "right" else M.t() @ M
+        lam, Q = torch.linalg.eigh(0.5 * (G + G.t()))
+        lam, Q = torch.flip(lam, [0]).clamp_min(0.0), torch.flip(Q, [1])
+        S = torch.sqrt(lam)
+    k = min(trunc_dim, int(lam.shape[0]))
+    if trunc_err > 0.0:
+        tail = torch.sqrt(torch.flip(torch.cumsum(torch.flip(lam, [0]), 0), [0]))
+        keep = int((tail > trunc_err).sum().item())
+        k = max(1, min(k, keep))
+    eps = float(torch.sqrt(lam[k:].sum())) if k < lam.shape[0] else 0.0
+    if method == "svd":
+        U, S, Vh = U[:, :k], S[:k], Vh[:k]
+        return (U, S[:, None] * Vh, S, eps) if direction == "right" else (U * S[None, :], Vh, S, eps)
+    Q = Q[:, :k].contiguous()
+    if direction == "right":
+        return Q, Q.t() @ M, S[:k], eps
+    return M @ Q, Q.t().contiguous(), S[:k], eps
+
+
+def gauge2(psi: FiniteMPS, pos, direction, ac2, trunc_dim, trunc_err, method="auto"):
+    """Truncated split of the two-site center (``gauge2!``, orthoview.jl:401-420)
     with ``normalize = true``; returns the discarded-weight 2-norm."""
     Dl, d1, Dr, d2 = ac2.shape
     M = T.fuse(ac2, 2)
-    U, S, Vh = torch.linalg.svd(M, full_matrices=False)
-    k = min(trunc_dim, S.shape[0])
-    if trunc_err > 0.0:
-        tail = torch.sqrt(torch.flip(torch.cumsum(torch.flip(S * S, [0]), 0), [0]))
-        keep = int((tail > trunc_err).sum().item())
-        k = max(1, min(k, keep))
-    eps = float(torch.linalg.vector_norm(S[k:])) if k < S.shape[0] else 0.0
-    U, S, Vh = U[:, :k], S[:k], Vh[:k]
-    S = S / torch.linalg.vector_norm(S)
-    Al = T.as_colmajor(T.unfuse(U, (Dl, d1, k)))
-    # rows of Vh run over (r, p2) with r fastest -> AR[s, p2, r]
-    ar = T.as_colmajor(T.unfuse(Vh, (k, Dr, d2)).permute(0, 2, 1))
+    A, B, S, eps = truncated_split(M, direction, trunc_dim, trunc_err, method)
+    k = A.shape[1]
+    # rows of B run over (r, p2) with r fastest -> tensor[s, p2, r]
     if direction == "right":
-        psi.AL[pos] = Al
-        psi.AC = T.as_colmajor(S[:, None, None] * ar)
+        psi.AL[pos] = T.as_colmajor(T.unfuse(A, (Dl, d1, k)))
+        c = T.unfuse(B, (k, Dr, d2)).permute(0, 2, 1)
+        psi.AC = T.as_colmajor(c / torch.linalg.vector_norm(c))
         psi.AR[pos] = None
         psi.AR[pos + 1] = None
         psi.center = pos + 1
     else:
-        psi.AR[pos + 1] = ar
-        psi.AC = T.as_colmajor(Al * S[None, None, :])
+        psi.AR[pos + 1] = T.as_colmajor(T.unfuse(B, (k, Dr, d2)).permute(0, 2, 1))
+        c = T.unfuse(A, (Dl, d1, k))
+        psi.AC = T.as_colmajor(c / torch.linalg.vector_norm(c))
         psi.AL[pos] = None
         psi.AL[pos + 1] = None
         psi.center = pos
@@ -173,7 +207,7 @@ def local_update(pos, direction, psi, H, alg: DMRG2, envs, eps_global, eps_trunc
         _, newac2, info = Heff.eigsolve(ac2, krylovdim=krylovdim, maxiter=maxiter, tol=tol,
                                         precision=alg.precision)
     stats["numops"] += info["numops"] + 1
-    eps_trunc = gauge2(psi, pos, direction, newac2, alg.trunc_dim, alg.trunc_err)
+    eps_trunc = gauge2(psi, pos, direction, newac2, alg.trunc_dim, alg.trunc_err, alg.gauge)
     numops = max(1, info["numops"])
     decay = (info["normres"] / eps_local) ** (1.0 / numops) if eps_local > 0 else 0.0
     decay_rate = min(max(decay, 1e-3), 0.999)
@@ -214,3 +248,60 @@ def find_groundstate(psi: FiniteMPS, H, alg: DMRG2, envs: FiniteEnvironments | N
         if not alg.exact_sweeps and eps_global <= max(alg.tol, max(eps_truncs)):
             break
     return psi, envs, eps_global, history
+
+
+def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None, use_graph: bool = True,
+                             verbosity: int = 0):
+    """DMRG2 sweeps of many chains of one length in lockstep (a parameter sweep).
+
+    Every bond step prepares the two-site Jordan operators of all instances,
+    solves their local problems with one batched eigensolve (``BatchedDerivative``:
+    one grouped launch per Krylov step for all instances) and splits every
+    instance's solution as ``local_update`` does (dmrg.jl:211-244).  ``solver``
+    fixes the Krylov settings shared by the batch (default Lanczos(tol=1e-10,
+    krylovdim=20, maxiter=100)).  Returns (psis, envs, history); history holds per
+    sweep the energies, the wall time and the applies of the slowest instance.
+    """
+    from .derivatives import BatchedDerivative
+    solver = solver or Lanczos(tol=1e-10, krylovdim=20, maxiter=100)
+    n = len(psis)
+    L = psis[0].L
+    if any(p.L != L for p in psis) or len(Hs) != n:
+        raise ValueError("find_groundstate_batched: instances must be chains of one length")
+    envs = [FiniteEnvironments(p, H) for p, H in zip(psis, Hs)]
+    precs = ("fp64",) if alg.precision == "fp64" else ("fp64", alg.precision)
+    bonds = [(p, "right") for p in range(L - 1)] + [(p, "left") for p in range(L - 3, -1, -1)]
+    history = []
+    for it in range(1, alg.maxiter + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        numops = 0
+        t_solve = 0.0
+        for pos, direction in bonds:
+            ops, xs = [], []
+            for psi, H, env in zip(psis, Hs, envs):
+                psi.move_center(pos if direction == "right" else pos + 1)
+                GL, GR = env.leftenv(pos, psi), env.rightenv(pos + 1, psi)
+                ops.append(prepare_operator(JordanMPO_AC2_Hamiltonian(GL, H[pos], H[pos + 1], GR), precisions=precs))
+                xs.append(psi.AC2(pos))
+            Bt = BatchedDerivative(ops)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            _, vecs, infos = Bt.eigsolve(xs, krylovdim=solver.krylovdim, maxiter=solver.maxiter, tol=solver.tol,
+                                         precision=alg.precision, use_graph=use_graph)
+            t_solve += time.perf_counter() - ts
+            numops += max(i["numops"] for i in infos)
+            for psi, v in zip(psis, vecs):
+                gauge2(psi, pos, direction, v, alg.trunc_dim, alg.trunc_err, alg.gauge)
+            Bt.free()
+            for op in ops:
+                op.free()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        energies = [expectation_value(p, H, e) for p, H, e in zip(psis, Hs, envs)]
+        history.append(dict(iter=it, energies=energies, time=dt, solve_time=t_solve, numops=numops))
+        if verbosity >= 1:
+            print(f"batched DMRG2 {it:3d}: {n} instances  time = {dt:.3f}s (solves {t_solve:.3f}s)  "
+                  f"applies = {numops}", flush=True)
+    return psis, envs, history
+

@@ -74,3 +74,30 @@ def test_tfim_L32_mixed_matches_fp64(mk, precision):
     print(f"E64={E64:.15f} Emixed={Em:.15f} exact={e_exact:.15f}", flush=True)
     assert abs(E64 - e_exact) / L < 1e-10
     assert abs(Em - E64) / L < 1e-10
+
+
+def test_eigh_gauge_matches_svd_gauge(mk):
+    """The Gram-eigh truncated split: isometries to 1e-13 and the same DMRG energy."""
+    import torch
+    from mpskit_amd.algorithms import truncated_split
+    g = torch.Generator(device="cuda").manual_seed(0)
+    M = torch.randn(600, 600, dtype=torch.float64, device="cuda", generator=g)
+    M = M * torch.logspace(0, -9, 600, dtype=torch.float64, device="cuda")[None, :]
+    for direction in ("right", "left"):
+        A1, B1, S1, eps = truncated_split(M, direction, 300, 0.0, "eigh")
+        A2, B2, S2, eps2 = truncated_split(M, direction, 300, 0.0, "svd")
+        iso = A1 if direction == "right" else B1.t()
+        err = float(torch.linalg.matrix_norm(iso.t() @ iso - torch.eye(300, dtype=torch.float64, device="cuda")))
+        assert err < 1e-13, (direction, err)
+        # same truncated matrix and discarded weight
+        assert float(torch.linalg.matrix_norm(A1 @ B1 - A2 @ B2)) < 1e-10 * float(torch.linalg.matrix_norm(M))
+        # Gram eigenvalues carry an absolute error ~ k * eps_mach * sigma_max^2
+        smax2 = float(S2[0]) ** 2
+        assert abs(eps ** 2 - eps2 ** 2) < 600 * 2.3e-16 * smax2
+    L, g_ = 12, 1.0
+    H = mk.FiniteMPOHamiltonian(mk.models.tfim(L, g_))
+    psi = mk.FiniteMPS.random(L, 2, 64, seed=0)
+    alg = mk.DMRG2(tol=1e-12, maxiter=6, trunc_dim=64, gauge="eigh", verbosity=0)
+    _, _, _, hist = mk.find_groundstate(psi, H, alg)
+    assert abs(hist[-1]["energy"] - B.tfim_open_energy(L, g_)) / L < 1e-10
+
