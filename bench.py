@@ -13,7 +13,19 @@ the FP64 and FP32 GPU paths, the speed-up over FP64, the relative error of
 each against the FP64 result, the GEMM roofline measured with HIP events, and
 the float64 numpy/OpenBLAS restatement of the same apply timed on host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--D 2048] [--no-cpu]
+The same line carries the DMRG2 sweep time at D=2048: one sweep of the
+Heisenberg chain (L=24, bonds up to 2048, random normalised start, seed 0) in
+FP64 and in the mixed path, with the energies of both.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--D 2048] [--no-cpu] [--no-sweep]
+
+Other workloads print their own JSON line:
+    --workload batched   20 TFIM chains (g = 0.1 .. 2.0, L = 48, D_i = 256 .. 1024)
+                         swept in lockstep with grouped launches; learned
+                         Davidson preconditioner against plain Lanczos
+    --workload u1        U(1) XXZ DMRG2, L = 64, D = 2048 over 21 charge sectors
+                         (charge-block grouped GEMMs, captured Krylov steps)
+    --workload vumps     VUMPS on the J1-J2 chain, cell 2, D = 1024
 """
 from __future__ import annotations
 
@@ -98,6 +110,202 @@ def cpu_baseline(D, budget_s=20.0):
                        f"(numpy/OpenBLAS float64: x*IC + BE*x + CB term), {el:.1f} s")
 
 
+def dmrg_sweep(mk, prec, L=24, D=2048):
+    """One DMRG2 sweep of the Heisenberg chain from a random state; (seconds, E/L, applies)."""
+    H = mk.FiniteMPOHamiltonian(mk.models.xxz(L, 1.0))
+    psi = mk.FiniteMPS.random(L, 2, D, seed=0)
+    alg = mk.DMRG2(tol=1e-12, maxiter=1, trunc_dim=D, precision=prec, exact_sweeps=True, verbosity=0,
+                   alg_eigsolve=mk.Lanczos(tol=1e-10, krylovdim=20, maxiter=50))
+    _, _, _, hist = mk.find_groundstate(psi, H, alg)
+    return hist[-1]["time"], hist[-1]["energy"] / L, hist[-1]["numops"]
+
+
+def traffic_from_profile(kernel, D):
+    """HBM bytes per launch of `kernel` at this D from the committed counter summary, if present."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        rec = json.loads(f.read_text()).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("D") != D:
+        return None
+    return rec.get("bytes_per_launch")
+
+
+def batched_workload(args):
+    """20 TFIM chains swept in lockstep (grouped launches), then the learned
+    Davidson preconditioner against plain Lanczos.
+
+    Two lockstep DMRG2 sweeps use a loose local tolerance (1e-6, as the
+    reference's adaptive tolerances do early on) in the mixed path; training
+    pairs come from Davidson solves of one rotating instance at every third
+    bond.  After the sweeps the local problems at four bonds along the chain are
+    solved to a float64 residual of 1e-10: all 20 instances by one batched
+    Lanczos (one grouped launch per Krylov step), and five instances spread over
+    g by per-instance Lanczos, Davidson with the diagonal correction and Davidson
+    with the trained network (applies and seconds of each).
+    """
+    import torch
+    import mpskit_amd as mk
+    L, n = args.L, 20
+    gs = [0.1 * (i + 1) for i in range(n)]
+    Ds = [int(round((256 + (768 * i) // 19) / 16.0)) * 16 for i in range(n)]
+    psis = [mk.FiniteMPS.random(L, 2, D, seed=100 + i) for i, D in enumerate(Ds)]
+    Hs = [mk.FiniteMPOHamiltonian(mk.models.tfim(L, g)) for g in gs]
+    net = mk.MLPPreconditioner(seed=1, capacity=400_000)
+    count = [0]
+
+    def hook(sweep, pos, direction, ops, xs):
+        count[0] += 1
+        print(f"# sweep {sweep} bond {pos} {direction}", file=sys.stderr, flush=True)
+        if count[0] % 3 == 0:
+            i = (count[0] // 3) % n
+            mk.davidson(ops[i], xs[i], maxdim=20, maxiter=200, tol=1e-8, mlp=net, mode="none", collect=True)
+
+    Dmax = max(Ds)
+    alg = mk.DMRG2(maxiter=2, trunc_dim=Dmax, precision="bf16x3")
+    loose = mk.Lanczos(tol=1e-6, krylovdim=16, maxiter=4)
+    _, envs, hist = mk.find_groundstate_batched(psis, Hs, alg, loose, use_graph=True, verbosity=1, on_bond=hook)
+    # parameters are selected on held-out local problems of three instances
+    def local(i, pos):
+        psi, H = psis[i], Hs[i]
+        psi.move_center(pos)
+        op = mk.prepare_operator(mk.JordanMPO_AC2_Hamiltonian(envs[i].leftenv(pos, psi), H[pos], H[pos + 1],
+                                                               envs[i].rightenv(pos + 1, psi)),
+                                 precisions=("fp64", "bf16x3"))
+        return op, psi.AC2(pos).clone()
+
+    held = [local(i, L // 2 - 1) for i in (3, 9, 15)]
+    rows = net.dataset_size
+    sel = net.select(held, rounds=4, steps=200, lr=1e-4)
+    print(f"# training pairs {rows}, selection {sel}", file=sys.stderr, flush=True)
+    for op, _ in held:
+        op.free()
+    st = {k: 0.0 for k in ("lanczos_s", "diag_s", "mlp_s", "batched_s")}
+    st.update({k: 0 for k in ("lanczos_ops", "diag_ops", "mlp_ops", "batched_ops", "solves")})
+    subset = (1, 5, 9, 13, 17)
+    for pos in (5, 17, 29, 41):
+        print(f"# compare solvers at bond {pos}", file=sys.stderr, flush=True)
+        probs = [local(i, pos) for i in range(n)]
+        Bt = mk.BatchedDerivative([op for op, _ in probs])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, _, infos = Bt.eigsolve([x for _, x in probs], krylovdim=20, maxiter=100, tol=1e-10, precision="bf16x3")
+        torch.cuda.synchronize()
+        st["batched_s"] += time.perf_counter() - t
+        st["batched_ops"] += max(i["numops"] for i in infos)
+        Bt.free()
+        for i in subset:
+            op, x = probs[i]
+            for mode in ("lanczos", "diag", "mlp"):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                if mode == "lanczos":
+                    _, _, info = op.eigsolve(x, krylovdim=20, maxiter=100, tol=1e-10)
+                else:
+                    _, _, info = mk.davidson(op, x, maxdim=20, maxiter=300, tol=1e-10, mlp=net, mode=mode)
+                torch.cuda.synchronize()
+                st[mode + "_s"] += time.perf_counter() - t
+                st[mode + "_ops"] += info["numops"]
+            st["solves"] += 1
+        for op, _ in probs:
+            op.free()
+    out = {
+        "metric": "batched DMRG2 sweep time, 20 TFIM chains in lockstep (grouped launches); learned Davidson "
+                  "vs Lanczos applies to a 1e-10 residual",
+        "value": hist[-1]["time"], "unit": "s/sweep", "n_gpus": 1, "steps": 2, "warmup": 0,
+        "ms_per_step": hist[-1]["time"] * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16x3", "data": "synthetic",
+        "config": {"workload": f"20 TFIM chains g=0.1..2.0, L={L}, D_i=256..1024 (multiples of 16), "
+                               "lockstep DMRG2, local tol 1e-6", "D_i": Ds},
+        "sweeps": [dict(iter=h["iter"], time=h["time"], solve_time=h["solve_time"], applies_slowest=h["numops"])
+                   for h in hist],
+        "energies_per_site": [E / L for E in hist[-1]["energies"]],
+        "batched_solve_1e-10": {"bonds": 4, "instances": n, "seconds": st["batched_s"],
+                                "applies_slowest": st["batched_ops"]},
+        "learned": {"training_pairs": rows, "selection_applies": sel, "local_solves": st["solves"],
+                    "instances": list(subset), "bonds": [5, 17, 29, 41],
+                    "applies": {k: st[k + "_ops"] for k in ("lanczos", "diag", "mlp")},
+                    "seconds": {k: st[k + "_s"] for k in ("lanczos", "diag", "mlp")}, "tol": 1e-10},
+    }
+    print(json.dumps(out), flush=True)
+
+
+U1_CENTER = [8, 8, 8, 16, 40, 64, 112, 168, 216, 256, 256, 256, 216, 168, 112, 64, 40, 16, 8, 8, 8]
+
+
+def u1_workload(args):
+    """U(1) XXZ DMRG2 (L = 64, Sz = 0, D = 2048 over 21 charge sectors with a
+    discretised Gaussian profile in multiples of 8): sweep time and applies in
+    float64 and in the mixed path, energies of both."""
+    import torch
+    import mpskit_amd as mk
+    L, nup = args.L if args.L != 48 else 64, None
+    nup = L // 2
+    prof = U1_CENTER
+
+    def dims(k, q):
+        # centre the profile on the bond's mean charge k/2
+        i = q - (k // 2) + len(prof) // 2
+        return prof[i] if 0 <= i < len(prof) else 0
+
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        psi = mk.U1FiniteMPS.random(L, nup, dims, seed=0)
+        H = mk.FiniteMPOHamiltonian(mk.models.xxz(L, 1.0))
+        print(f"# u1 {prec} bonds {psi.bond_dims()}", file=sys.stderr, flush=True)
+        hist = mk.find_groundstate_u1(psi, H, trunc_dim=2048, sweeps=args.u1_sweeps, precision=prec, tol=1e-10,
+                                      verbosity=1)
+        res[prec] = dict(sweeps=[dict(time=h["time"], solve_time=h["solve_time"], applies=h["numops"],
+                                      energy_per_site=h["energy"] / L) for h in hist],
+                         mid_bond=hist[-1]["mid_bond"])
+    last = {p: r["sweeps"][-1] for p, r in res.items()}
+    out = {
+        "metric": "U(1) XXZ DMRG2 sweep time at D=2048 over 21 charge sectors (charge-block grouped GEMMs)",
+        "value": last["bf16x3"]["time"], "unit": "s/sweep", "n_gpus": 1, "steps": args.u1_sweeps, "warmup": 0,
+        "ms_per_step": last["bf16x3"]["time"] * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16x3", "data": "synthetic",
+        "config": {"workload": f"U(1) XXZ DMRG2, L={L}, Sz=0, D=2048 over 21 sectors {prof}"},
+        "fp64": res["fp64"], "bf16x3": res["bf16x3"],
+        "energy_diff_per_site": abs(last["fp64"]["energy_per_site"] - last["bf16x3"]["energy_per_site"]),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def vumps_workload(args):
+    """VUMPS on the infinite J1-J2 chain (J2/J1 = 0.24, unit cell 2, w = 8) at bond
+    dimension D: seconds per iteration in float64 and in the mixed path (FP64
+    environment normalisation, gauge and energies in both), over a fixed number
+    of iterations from the same random state."""
+    import torch
+    import mpskit_amd as mk
+    from mpskit_amd import vumps as V
+    D, iters = args.vumps_D, args.vumps_iters
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        H = V.InfiniteHamiltonian([mk.models.j1j2_bulk(1.0, 0.24)] * 2)
+        psi = V.InfiniteMPS.random(2, 2, D, seed=7)
+        print(f"# vumps {prec} D={D}", file=sys.stderr, flush=True)
+        alg = V.VUMPS(tol=1e-12, maxiter=iters, precision=prec, verbosity=2)
+        _, _, eps, hist = V.find_groundstate_vumps(psi, H, alg)
+        per = [h["time"] for h in hist[1:]] or [hist[0]["time"]]
+        res[prec] = dict(seconds_per_iteration=sum(per) / len(per), energy_per_site=hist[-1]["energy"],
+                         galerkin=hist[-1]["eps"], iterations=len(hist))
+    out = {
+        "metric": "VUMPS time per iteration, infinite J1-J2 chain (J2/J1=0.24, cell 2, w=8)",
+        "value": res["bf16x3"]["seconds_per_iteration"], "unit": "s/iteration", "n_gpus": 1, "steps": iters,
+        "warmup": 1, "ms_per_step": res["bf16x3"]["seconds_per_iteration"] * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3", "data": "synthetic",
+        "config": {"workload": f"VUMPS J1-J2 J2/J1=0.24 unit cell 2 D={D}, random start seed 7, {iters} iterations"},
+        "fp64": res["fp64"], "bf16x3": res["bf16x3"],
+        "speedup": res["fp64"]["seconds_per_iteration"] / res["bf16x3"]["seconds_per_iteration"],
+        "energy_diff_per_site": abs(res["fp64"]["energy_per_site"] - res["bf16x3"]["energy_per_site"]),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +315,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
                     help="time one precision only (for profiler runs); prints no JSON line")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048")
+    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1"], default="apply")
+    ap.add_argument("--vumps-D", type=int, default=1024)
+    ap.add_argument("--u1-sweeps", type=int, default=2)
+    ap.add_argument("--vumps-iters", type=int, default=6)
+    ap.add_argument("--L", type=int, default=48, help="chain length of the batched workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device-to-device five times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -126,6 +340,16 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print("# copied 5 x 1 GiB", file=sys.stderr)
+        return
+
+    if args.workload == "batched":
+        batched_workload(args)
+        return
+    if args.workload == "vumps":
+        vumps_workload(args)
+        return
+    if args.workload == "u1":
+        u1_workload(args)
         return
 
     D, d = args.D, 2
@@ -185,14 +409,28 @@ def main():
         "rel_err_vs_fp64": {"bf16x3": res["bf16x3"]["err"], "fp32": res["fp32"]["err"]},
         "tflops": {p: F / (r["ms"] * 1e-3) / 1e12 for p, r in res.items()},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_TF["bf16x3"], "unit": "TFLOP/s",
-                     "frac": ach / PEAK_TF["bf16x3"], "traffic": None,
-                     "kernel": "gemm_mixed_kernel<128,128,BF16X3>",
+                     "frac": ach / PEAK_TF["bf16x3"],
+                     "traffic": traffic_from_profile("gemm_mixed_big_kernel<BF16X3>", D),
+                     "kernel": "gemm_mixed_big_kernel<BF16X3> (256x256 tiles, 8 waves)",
                      "peak_note": "BF16 dense 2516.6 TF/s over 3 products per float64 product"},
         "roofline_fp64": {"bound": "mfma", "achieved": F / (res["fp64"]["gemm_ms"] * 1e-3) / 1e12,
                           "peak": PEAK_TF["fp64"], "unit": "TFLOP/s",
                           "frac": F / (res["fp64"]["gemm_ms"] * 1e-3) / 1e12 / PEAK_TF["fp64"],
+                          "traffic": traffic_from_profile("gemm_f64_kernel<128,128>", D),
                           "kernel": "gemm_f64_kernel<128,128>"},
     }
+    if not args.no_sweep:
+        sw = {}
+        for prec in ("fp64", "bf16x3"):
+            print(f"# DMRG2 sweep {prec} ...", file=sys.stderr, flush=True)
+            sec, e, nops = dmrg_sweep(mk, prec)
+            sw[prec] = dict(seconds=sec, energy_per_site=e, applies=nops)
+        out["dmrg_sweep"] = {"workload": "DMRG2, Heisenberg chain L=24, bonds up to 2048, one sweep from a random "
+                                         "normalised state (seed 0), Lanczos tol 1e-10",
+                             "fp64": sw["fp64"], "bf16x3": sw["bf16x3"],
+                             "speedup": sw["fp64"]["seconds"] / sw["bf16x3"]["seconds"],
+                             "energy_diff_per_site": abs(sw["fp64"]["energy_per_site"] -
+                                                         sw["bf16x3"]["energy_per_site"])}
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(D)

@@ -122,6 +122,22 @@ mk_status mk_prepare_ac2_jordan(mk_ctx ctx, const double* GL, const double* W1,
                                 int64_t wl, int64_t d1, int64_t wm, int64_t d2,
                                 int64_t wr, int64_t Dr, int precisions, mk_op* out);
 
+/* ---- U(1) charge blocks ------------------------------------------------------
+ * Counterpart of TensorKit's block-sparse U(1) storage under the same
+ * derivative operators (the reference applies one GEMM per fusion-tree block,
+ * src/utility/utility.jl:285-302).  Tensors stay dense arrays in a
+ * charge-sorted basis (each bond index space is a concatenation of charge
+ * sectors); this call tells a prepared nearest-neighbour two-site Jordan
+ * operator which blocks conserve charge, after which its applies (and batches
+ * holding it) compute only those blocks, all of them in one grouped persistent
+ * GEMM launch.  Left bond sectors are [loff[s], loff[s+1]) with charge lq[s]
+ * (increasing), right bond sectors likewise, pq[h] the charge of physical
+ * state h; x[f,h,i,j] may be nonzero only if lq(f) + pq(h) + pq(j) = rq(i),
+ * and y is returned with the same structure (elements outside the blocks are
+ * the physical term's).  Mixed precisions need every offset a multiple of 8. */
+mk_status mk_op_set_charge_blocks(mk_op op, int nl, const int64_t* loff, const int* lq, int nr,
+                                  const int64_t* roff, const int* rq, const int* pq);
+
 /* ---- derivative operators (apply) ---------------------------------------- */
 
 /* y = alpha * H(x) + beta * y using arithmetic `precision` (which must have

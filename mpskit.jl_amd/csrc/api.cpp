@@ -62,7 +62,6 @@ mk_status Ctx::small(size_t nbytes, void** out) {
   return MK_OK;
 }
 
-static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 // Carve several regions out of the context arena.
 mk_status carve(Ctx* ctx, std::initializer_list<std::pair<size_t, void**>> regions) {

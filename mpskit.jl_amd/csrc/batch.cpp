@@ -32,8 +32,6 @@ void lanczos_lowest(const std::vector<double>& G, int ld, int k, double& lam, st
 
 namespace {
 
-int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
-
 struct PrecPlan {
   bool ready = false;
   GroupPlan gemm;
@@ -86,8 +84,9 @@ struct Batch {
 namespace {
 
 bool nn_jordan(const Op* op) {
+  // charge-blocked operators leave y outside their blocks to the physical pass
   return op->kind == OP_AC2_JOR && !op->CA.present() && op->AB.empty() && !op->gen.present() &&
-         (op->IC.present() || op->BE.present());
+         (op->IC.present() || op->BE.present()) && (op->blocks.empty() || op->hasP);
 }
 
 }  // namespace
@@ -104,6 +103,7 @@ mk_status Batch::ensure_plan(int prec) {
   const bool mixed = prec != MK_FP64;
   std::vector<ProbSpec> probs(n);
   std::vector<SplitItem> items;
+  std::vector<ProbSpec> block_specs;
   int64_t sbytes = 0;
   for (int i = 0; i < n; ++i) {
     Op* op = ops[i];
@@ -117,6 +117,10 @@ mk_status Batch::ensure_plan(int prec) {
     p.beta = op->hasP ? 1.0 : 0.0;
     const void* x = rel_ptr(1, xoff(i));
     if (!mixed) {
+      if (!op->blocks.empty()) {
+        block_probs(op, prec, xoff(i), 0, 0, 1.0, block_specs);
+        continue;
+      }
       if (op->IC.present()) p.add(x, M, op->IC.f64(), N, N);
       if (op->BE.present()) p.add(op->BE.f64(), M, x, M, M);
       continue;
@@ -144,6 +148,15 @@ mk_status Batch::ensure_plan(int prec) {
       p.add(bf ? m.hi.ptr : m.f32.ptr, m.kpad, rel_ptr(3, b_hi), ldb, M, bf ? m.lo.ptr : nullptr,
             bf ? rel_ptr(3, b_lo) : nullptr);
     }
+    if (!op->blocks.empty()) block_probs(op, prec, xoff(i), a_hi, b_hi, 1.0, block_specs);
+  }
+  // operators with charge blocks contribute their block problems instead of the dense one
+  {
+    std::vector<ProbSpec> kept;
+    for (int i = 0; i < n; ++i)
+      if (ops[i]->blocks.empty()) kept.push_back(probs[i]);
+    kept.insert(kept.end(), block_specs.begin(), block_specs.end());
+    probs.swap(kept);
   }
   if (!items.empty()) {
     MK_TRY(pp.split.alloc(items.size() * sizeof(SplitItem)));
@@ -159,7 +172,10 @@ mk_status Batch::ensure_plan(int prec) {
     MK_TRY(scratch.alloc((size_t)sbytes));
   }
   const int64_t base_bytes[NBASE] = {0, total * 8, total * 8, sbytes};
-  MK_TRY(group_plan(ctx, probs.data(), n, prec, GemmShape{}, base_bytes, -1, &pp.gemm));
+  for (int i = 0; i < n; ++i)
+    MK_REQUIRE(prec == MK_FP64 || ops[i]->blocks.empty() || ops[i]->blocks_aligned, MK_NOT_SUPPORTED,
+               "batch: charge blocks of operator %d are not multiples of 8 (mixed precision)", i);
+  MK_TRY(group_plan(ctx, probs.data(), (int)probs.size(), prec, GemmShape{}, base_bytes, -1, &pp.gemm));
   pp.ready = true;
   return MK_OK;
 }

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <vector>
 
+#include "group.h"
 #include "kernels.h"
 #include "ops.h"
 
@@ -195,7 +196,94 @@ Side mat_side(const OpMat& m, int prec) {
 }
 Side view_side(const MixedView& v) { return {v.hi, v.lo, v.kpad}; }
 
+}  // namespace
+
+void block_probs(const Op* op, int prec, int64_t xoff, int64_t sa, int64_t sb, double beta,
+                 std::vector<ProbSpec>& out) {
+  const int64_t M = op->Dl * op->d1, N = op->Dr * op->d2;
+  const int64_t kpA = pad8(N), kpB = pad8(M);
+  const int64_t eb = prec == MK_BF16X3 ? 2 : 4;
+  const int64_t loA = align256(M * kpA * eb), loB = align256(N * kpB * eb);
+  for (const Op::BlockProb& bp : op->blocks) {
+    ProbSpec p;
+    p.M = bp.m;
+    p.N = bp.n;
+    p.C = const_cast<double*>(static_cast<const double*>(rel_ptr(2, xoff + (bp.r0 + bp.c0 * M) * 8)));
+    p.ldc = M;
+    p.alpha = 1.0;
+    p.beta = beta;
+    for (int q = 0; q < bp.nseg; ++q) {
+      const Op::BlockSeg& sg = bp.seg[q];
+      if (prec == MK_FP64) {
+        if (sg.kind == 'I')
+          p.add(rel_ptr(1, xoff + (bp.r0 + sg.k0 * M) * 8), M, op->IC.f64() + sg.k0 + bp.c0 * N, N, sg.k);
+        else
+          p.add(op->BE.f64() + bp.r0 + sg.k0 * M, M, rel_ptr(1, xoff + (sg.k0 + bp.c0 * M) * 8), M, sg.k);
+        continue;
+      }
+      const bool bf = prec == MK_BF16X3;
+      if (sg.kind == 'I') {
+        const OpMat& m = op->IC;
+        const char* hi = static_cast<const char*>(bf ? m.hi.ptr : m.f32.ptr) + (bp.c0 * m.kpad + sg.k0) * eb;
+        const char* lo = bf ? static_cast<const char*>(m.lo.ptr) + (bp.c0 * m.kpad + sg.k0) * eb : nullptr;
+        const int64_t a = sa + (bp.r0 * kpA + sg.k0) * eb;
+        p.add(rel_ptr(3, a), kpA, hi, m.kpad, sg.k, bf ? rel_ptr(3, a + loA) : nullptr, lo);
+      } else {
+        const OpMat& m = op->BE;
+        const char* hi = static_cast<const char*>(bf ? m.hi.ptr : m.f32.ptr) + (bp.r0 * m.kpad + sg.k0) * eb;
+        const char* lo = bf ? static_cast<const char*>(m.lo.ptr) + (bp.r0 * m.kpad + sg.k0) * eb : nullptr;
+        const int64_t b = sb + (bp.c0 * kpB + sg.k0) * eb;
+        p.add(hi, m.kpad, rel_ptr(3, b), kpB, sg.k, lo, bf ? rel_ptr(3, b + loB) : nullptr);
+      }
+    }
+    out.push_back(p);
+  }
+}
+
+namespace {
+
+// Charge-block apply: y = alpha H x + beta y with H restricted to the blocks.
+mk_status apply_blocks(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  Ctx* ctx = op->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t Dl = op->Dl, Dr = op->Dr, d1 = op->d1, d2 = op->d2;
+  const int64_t M = Dl * d1, N = Dr * d2, n = op->x_elems;
+  MK_REQUIRE(prec == MK_FP64 || op->blocks_aligned, MK_NOT_SUPPORTED,
+             "charge-block apply: mixed precision needs sector ranges in multiples of 8");
+  const size_t va = prec == MK_FP64 ? 0 : view_bytes(M, N, 'A', prec);
+  const size_t vb = prec == MK_FP64 ? 0 : view_bytes(M, N, 'B', prec);
+  void *mv = nullptr, *tmp = nullptr;
+  MK_TRY(carve(ctx, {{va + vb, &mv}, {alpha != 1.0 ? (size_t)n * 8 : 0, &tmp}}));
+  double* out = alpha != 1.0 ? static_cast<double*>(tmp) : y;
+  const double b0 = alpha != 1.0 ? 0.0 : beta;
+  // every element: physical term (or the beta scaling); then the blocks accumulate
+  if (op->hasP)
+    MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, 1.0, b0, out, st));
+  else
+    MK_TRY(k_axpby(0.0, out, b0, out, n, st));
+  if (prec != MK_FP64) {
+    MixedView xa, xb;
+    MK_TRY(convert_view(ctx, x, M, N, M, 'A', prec, &xa, mv));
+    MK_TRY(convert_view(ctx, x, M, N, M, 'B', prec, &xb, static_cast<char*>(mv) + va));
+  }
+  auto* plan = static_cast<GroupPlan*>(op->bplan[prec].get());
+  if (!plan) {
+    std::vector<ProbSpec> probs;
+    block_probs(op, prec, 0, 0, (int64_t)va, 1.0, probs);
+    auto gp = std::make_shared<GroupPlan>();
+    const int64_t bb[NBASE] = {0, n * 8, n * 8, (int64_t)(va + vb)};
+    MK_TRY(group_plan(ctx, probs.data(), (int)probs.size(), prec, GemmShape{}, bb, -1, gp.get()));
+    op->bplan[prec] = gp;
+    plan = gp.get();
+  }
+  const void* base[NBASE] = {nullptr, x, out, mv};
+  MK_TRY(plan->run(base, st));
+  if (alpha != 1.0) MK_TRY(k_axpby(alpha, out, beta, y, n, st));
+  return MK_OK;
+}
+
 mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  if (!op->blocks.empty()) return apply_blocks(op, x, y, alpha, beta, prec);
   Ctx* ctx = op->ctx;
   const int64_t Dl = op->Dl, Dr = op->Dr, d1 = op->d1, d2 = op->d2;
   const int64_t M = Dl * d1, N = Dr * d2;
@@ -338,6 +426,75 @@ mk_status op_diagonal(Op* op, double* diag, hipStream_t st) {
 }  // namespace mk
 
 using namespace mk;
+
+// ---------------------------------------------------------------------------
+// C ABI: charge blocks
+// ---------------------------------------------------------------------------
+
+extern "C" mk_status mk_op_set_charge_blocks(mk_op h, int nl, const int64_t* loff, const int* lq, int nr,
+                                             const int64_t* roff, const int* rq, const int* pq) {
+  MK_REQUIRE(h != nullptr && loff && lq && roff && rq && pq, MK_BAD_ARG, "charge blocks: null argument");
+  Op* op = reinterpret_cast<Op*>(h);
+  MK_REQUIRE(op->kind == OP_AC2_JOR && !op->CA.present() && op->AB.empty() && !op->gen.present(),
+             MK_NOT_SUPPORTED, "charge blocks: only nearest-neighbour two-site Jordan operators");
+  MK_REQUIRE(op->d1 == op->d2 && op->d1 + op->d2 <= MAX_SEG, MK_NOT_SUPPORTED,
+             "charge blocks: equal physical spaces with d1 + d2 <= %d", MAX_SEG);
+  const int64_t Dl = op->Dl, Dr = op->Dr, d = op->d1;
+  MK_REQUIRE(nl >= 1 && nr >= 1 && loff[0] == 0 && loff[nl] == Dl && roff[0] == 0 && roff[nr] == Dr, MK_BAD_SHAPE,
+             "charge blocks: sector offsets must cover [0, Dl) and [0, Dr)");
+  for (int s = 0; s < nl; ++s)
+    MK_REQUIRE(loff[s] < loff[s + 1] && (s == 0 || lq[s] > lq[s - 1]), MK_BAD_SHAPE,
+               "charge blocks: left sectors must be non-empty with increasing charges");
+  for (int s = 0; s < nr; ++s)
+    MK_REQUIRE(roff[s] < roff[s + 1] && (s == 0 || rq[s] > rq[s - 1]), MK_BAD_SHAPE,
+               "charge blocks: right sectors must be non-empty with increasing charges");
+  auto find = [](int n, const int* q, int c) {
+    for (int s = 0; s < n; ++s)
+      if (q[s] == c) return s;
+    return -1;
+  };
+  struct Part { int64_t o, len; };
+  std::vector<Op::BlockProb> blocks;
+  bool aligned = Dl % 8 == 0 && Dr % 8 == 0;
+  for (int s = 0; s <= nl; ++s) aligned = aligned && loff[s] % 8 == 0;
+  for (int s = 0; s <= nr; ++s) aligned = aligned && roff[s] % 8 == 0;
+  // middle charges c = q_left + q_phys(h) = q_right - q_phys(j)
+  std::vector<int> cs;
+  for (int s = 0; s < nl; ++s)
+    for (int64_t hh = 0; hh < d; ++hh) cs.push_back(lq[s] + pq[hh]);
+  std::sort(cs.begin(), cs.end());
+  cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+  double fl = 0.0;
+  for (int c : cs) {
+    std::vector<Part> rows, cols;
+    for (int64_t hh = 0; hh < d; ++hh) {
+      const int s = find(nl, lq, c - pq[hh]);
+      if (s >= 0) rows.push_back({loff[s] + Dl * hh, loff[s + 1] - loff[s]});
+    }
+    for (int64_t j = 0; j < d; ++j) {
+      const int t = find(nr, rq, c + pq[j]);
+      if (t >= 0) cols.push_back({roff[t] + Dr * j, roff[t + 1] - roff[t]});
+    }
+    if (rows.empty() || cols.empty()) continue;
+    for (const Part& rp : rows)
+      for (const Part& cp : cols) {
+        Op::BlockProb bp{};
+        bp.r0 = rp.o; bp.m = rp.len; bp.c0 = cp.o; bp.n = cp.len;
+        if (op->IC.present())
+          for (const Part& kp : cols) bp.seg[bp.nseg++] = {'I', kp.o, kp.len};
+        if (op->BE.present())
+          for (const Part& kp : rows) bp.seg[bp.nseg++] = {'E', kp.o, kp.len};
+        for (int q = 0; q < bp.nseg; ++q) fl += 2.0 * bp.m * bp.n * bp.seg[q].k;
+        blocks.push_back(bp);
+      }
+  }
+  MK_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
+  op->blocks = std::move(blocks);
+  op->blocks_aligned = aligned;
+  for (auto& b : op->bplan) b.reset();
+  op->flops = fl + (op->hasP ? 2.0 * (double)op->x_elems * op->d1 * op->d2 : 0.0);
+  return MK_OK;
+}
 
 // ---------------------------------------------------------------------------
 // C ABI: prepares

@@ -2,6 +2,7 @@
 #pragma once
 #include <atomic>
 #include <initializer_list>
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -39,6 +40,7 @@ struct Ctx {
 };
 
 mk_status carve(Ctx* ctx, std::initializer_list<std::pair<size_t, void**>> regions);
+inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 int64_t pad8(int64_t k);
 
 // An operator matrix used as GEMM operand: role 'A' (rows = M, cols = K) or
@@ -103,7 +105,30 @@ struct Op {
 
   // Jordan one-site terms
   OpMat Cp, Bp;
+
+  // U(1) charge blocks of a nearest-neighbour two-site operator: when set,
+  // applies only compute the output blocks that conserve charge, each a sum of
+  // x[r, k] IC[k, c] ('I') and BE[r, k] x[k, c] ('E') products over the
+  // charge-compatible ranges k of the dense charge-sorted layout
+  struct BlockSeg {
+    char kind;
+    int64_t k0, k;
+  };
+  struct BlockProb {
+    int64_t r0, m, c0, n;
+    int nseg = 0;
+    BlockSeg seg[MAX_SEG];
+  };
+  std::vector<BlockProb> blocks;
+  bool blocks_aligned = false;       // every range a multiple of 8 (mixed precisions allowed)
+  std::shared_ptr<void> bplan[3];    // GroupPlan per precision, built on first use
 };
+
+// Grouped-GEMM problems of an operator's charge blocks: x at slot 1 + xoff, y at
+// slot 2 + xoff (bytes); mixed views of x at slot 3 + sa (left-factor view, row
+// pitch pad8(N)) and slot 3 + sb (right-factor view, pitch pad8(M)).
+void block_probs(const Op* op, int prec, int64_t xoff, int64_t sa, int64_t sb, double beta,
+                 std::vector<ProbSpec>& out);
 
 mk_status op_apply(Op* op, const double* x, double* y, double alpha, double beta, int prec);
 

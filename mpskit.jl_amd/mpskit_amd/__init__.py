@@ -27,9 +27,11 @@ from .algorithms import (
     DMRG2, AdaptiveKrylov, Davidson, Lanczos, expectation_value, find_groundstate, find_groundstate_batched,
 )
 from .learned import MLPPreconditioner, davidson
+from .u1 import U1Bond, U1FiniteMPS, find_groundstate_u1, set_charge_blocks
 from .environments import FiniteEnvironments, environments
 from .states import FiniteMPOHamiltonian, FiniteMPS
 
-__all__ += ["Davidson", "MLPPreconditioner", "davidson", "find_groundstate_batched"]
+__all__ += ["Davidson", "MLPPreconditioner", "davidson", "find_groundstate_batched", "U1Bond", "U1FiniteMPS",
+            "find_groundstate_u1", "set_charge_blocks"]
 __all__ += ["DMRG2", "AdaptiveKrylov", "Lanczos", "expectation_value", "find_groundstate", "FiniteEnvironments",
             "environments", "FiniteMPOHamiltonian", "FiniteMPS"]

@@ -55,6 +55,8 @@ _SIGS = {
     "mk_prepare_ac2_generic": ([_P, _P, _P, _P, _P] + [_I64] * 9 + [_INT, C.POINTER(_P)], _INT),
     "mk_prepare_ac_jordan": ([_P, _P, _P, _P] + [_I64] * 5 + [_INT, C.POINTER(_P)], _INT),
     "mk_prepare_ac2_jordan": ([_P, _P, _P, _P, _P] + [_I64] * 7 + [_INT, C.POINTER(_P)], _INT),
+    "mk_op_set_charge_blocks": ([_P, _INT, C.POINTER(_I64), C.POINTER(_INT), _INT, C.POINTER(_I64), C.POINTER(_INT),
+                                 C.POINTER(_INT)], _INT),
     "mk_apply": ([_P, _P, _P, _D, _D, _INT], _INT),
     "mk_op_info": ([_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_D)], _INT),
     "mk_op_free": ([_P], _INT),

@@ -251,7 +251,7 @@ def find_groundstate(psi: FiniteMPS, H, alg: DMRG2, envs: FiniteEnvironments | N
 
 
 def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None, use_graph: bool = True,
-                             verbosity: int = 0):
+                             verbosity: int = 0, on_bond=None):
     """DMRG2 sweeps of many chains of one length in lockstep (a parameter sweep).
 
     Every bond step prepares the two-site Jordan operators of all instances,
@@ -261,6 +261,9 @@ def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None
     fixes the Krylov settings shared by the batch (default Lanczos(tol=1e-10,
     krylovdim=20, maxiter=100)).  Returns (psis, envs, history); history holds per
     sweep the energies, the wall time and the applies of the slowest instance.
+    ``on_bond(sweep, pos, direction, ops, xs)``, if given, sees every bond's
+    prepared operators and start vectors before the batched solve (its time is
+    excluded from the sweep time).
     """
     from .derivatives import BatchedDerivative
     solver = solver or Lanczos(tol=1e-10, krylovdim=20, maxiter=100)
@@ -284,6 +287,12 @@ def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None
                 GL, GR = env.leftenv(pos, psi), env.rightenv(pos + 1, psi)
                 ops.append(prepare_operator(JordanMPO_AC2_Hamiltonian(GL, H[pos], H[pos + 1], GR), precisions=precs))
                 xs.append(psi.AC2(pos))
+            if on_bond is not None:
+                torch.cuda.synchronize()
+                th = time.perf_counter()
+                on_bond(it, pos, direction, ops, xs)
+                torch.cuda.synchronize()
+                t0 += time.perf_counter() - th
             Bt = BatchedDerivative(ops)
             torch.cuda.synchronize()
             ts = time.perf_counter()
