@@ -410,8 +410,8 @@ def main():
         "tflops": {p: F / (r["ms"] * 1e-3) / 1e12 for p, r in res.items()},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_TF["bf16x3"], "unit": "TFLOP/s",
                      "frac": ach / PEAK_TF["bf16x3"],
-                     "traffic": traffic_from_profile("gemm_mixed_big_kernel<BF16X3>", D),
-                     "kernel": "gemm_mixed_big_kernel<BF16X3> (256x256 tiles, 8 waves)",
+                     "traffic": traffic_from_profile("gemm_mixed_big2_kernel<BF16X3>", D),
+                     "kernel": "gemm_mixed_big2_kernel<BF16X3> (256x256 tiles, 8 waves)",
                      "peak_note": "BF16 dense 2516.6 TF/s over 3 products per float64 product"},
         "roofline_fp64": {"bound": "mfma", "achieved": F / (res["fp64"]["gemm_ms"] * 1e-3) / 1e12,
                           "peak": PEAK_TF["fp64"], "unit": "TFLOP/s",

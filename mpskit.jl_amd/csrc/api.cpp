@@ -117,23 +117,27 @@ mk_status OpMat::make_mixed(int precisions, hipStream_t st) {
 // ---------------------------------------------------------------------------
 
 // A transient float64 view converted for a mixed apply (x, xR...).
-mk_status convert_view(Ctx* ctx, const double* M, int64_t rows, int64_t cols, int64_t ld, char role,
-                       int prec, MixedView* out, void* mem) {
+MixedView view_layout(int64_t rows, int64_t cols, char role, int prec, void* mem) {
   const bool kfast = (role == 'B');
   const int64_t K = kfast ? rows : cols;
   const int64_t MN = kfast ? cols : rows;
-  out->kpad = pad8(K);
-  const size_t n = (size_t)MN * out->kpad;
+  MixedView v;
+  v.kpad = pad8(K);
+  const size_t n = (size_t)MN * v.kpad;
   char* p = static_cast<char*>(mem);
-  if (prec == MK_BF16X3) {
-    out->hi = p;
-    out->lo = p + align256(n * 2);
+  v.hi = p;
+  v.lo = prec == MK_BF16X3 ? p + align256(n * 2) : nullptr;
+  return v;
+}
+
+mk_status convert_view(Ctx* ctx, const double* M, int64_t rows, int64_t cols, int64_t ld, char role,
+                       int prec, MixedView* out, void* mem) {
+  const bool kfast = (role == 'B');
+  *out = view_layout(rows, cols, role, prec, mem);
+  if (prec == MK_BF16X3)
     MK_TRY(k_split_bf16(M, rows, cols, ld, kfast ? 1 : 0, out->kpad, out->hi, out->lo, ctx->stream));
-  } else {
-    out->hi = p;
-    out->lo = nullptr;
+  else
     MK_TRY(k_to_f32(M, rows, cols, ld, kfast ? 1 : 0, out->kpad, static_cast<float*>(out->hi), ctx->stream));
-  }
   return MK_OK;
 }
 

@@ -57,6 +57,7 @@ struct GemmGroup {
   int32_t nprob;
   int32_t total_tiles;
   const char* base[NBASE];
+  int32_t rows256_fit;   // 256 rows of every mixed operand span < 2 GiB (32-bit buffer offsets)
 };
 
 constexpr uint64_t REL_MASK = (1ull << 62) - 1;

@@ -34,15 +34,16 @@ constexpr int ROWP = ROWB + 16;   // padded LDS row (80 B: conflict-free 16-B re
 
 enum Mode { M_BF16X3 = 0, M_F32 = 1 };
 
-// Kernel family for large problems, read once from MK_GEMM_VARIANT ("big",
-// "pipe", "tile"; default "big"): lets benchmarks and tests pin one family.
-enum Variant { V_PIPE = 0, V_BIG = 1, V_TILE = 2 };
+// Kernel family for large problems, read once from MK_GEMM_VARIANT ("big2",
+// "big", "pipe", "tile"; default "big2"): lets benchmarks and tests pin one family.
+enum Variant { V_PIPE = 0, V_BIG = 1, V_TILE = 2, V_BIG2 = 3 };
 int variant() {
   static const int v = [] {
     const char* e = getenv("MK_GEMM_VARIANT");
     if (e && !strcmp(e, "pipe")) return (int)V_PIPE;
     if (e && !strcmp(e, "tile")) return (int)V_TILE;
-    return (int)V_BIG;
+    if (e && !strcmp(e, "big")) return (int)V_BIG;
+    return (int)V_BIG2;
   }();
   return v;
 }
@@ -377,6 +378,169 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(Src src) {
   }  // tiles
 }
 
+// Raw (stride 0) buffer resource over p with num_records = 2^31 - 1 bytes and
+// the gfx9 data-format word; offsets at or past num_records load zeros.
+constexpr unsigned OOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+
+// Same tile, waves and LDS layout as gemm_mixed_big_kernel, with a lighter
+// load path: the segment's operand descriptors are read once per segment (not
+// once per K step), each operand is addressed through a buffer resource based
+// at the tile's first row, and rows or K chunks outside the operand get a
+// buffer offset past num_records (the hardware returns zeros), so a K step
+// issues its loads without scalar loads or branches.  Needs 256 rows of every
+// operand within 2 GiB of the row base (checked on the host).
+template <int MODE, class Src>
+__global__ __launch_bounds__(BNT) void gemm_mixed_big2_kernel(Src src) {
+  constexpr int NPART = MODE == M_BF16X3 ? 2 : 1;
+  constexpr int EB = MODE == M_BF16X3 ? 2 : 4;
+  constexpr int WM = 128, WN = 64;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int CH = BIG * ROWB / 16 / BNT;   // 16-B chunks per thread per operand part
+  constexpr int KSTEP = ROWB / EB;
+
+  __shared__ __attribute__((aligned(16))) char As[2][NPART][BIG * ROWB];
+  __shared__ __attribute__((aligned(16))) char Bs[2][NPART][BIG * ROWB];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+  const int cq = threadIdx.x & 3;             // 16-B chunk of the 64-B row slab
+  int crow[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) crow[c] = (threadIdx.x + c * BNT) >> 2;
+  const int aoff = swz(wm0 + li, lg);
+  const int boff = swz(wn0 + li, lg);
+
+  for (int t = src_first(src); t < src.total_tiles; t += src_stride(src)) {
+  const GemmProb& P = src_prob(src, t);
+  int tm, tn;
+  tile_coords(P, t - P.tile_start, tm, tn);
+  const int m0 = tm * BIG, n0 = tn * BIG;
+  const int nseg = P.nseg;
+
+  fx4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+
+  // per-segment load state
+  __amdgpu_buffer_rsrc_t rA[NPART], rB[NPART];
+  unsigned vA[CH], vB[CH];
+  int Kb = 0, kend = 0;
+  auto set_seg = [&](int s) {
+    const GemmSeg& S = P.seg[s];
+    const int64_t ldab = S.lda * EB, ldbb = S.ldb * EB;
+    rA[0] = raw_rsrc(static_cast<const char*>(resolve(src, S.A)) + (int64_t)m0 * ldab);
+    rB[0] = raw_rsrc(static_cast<const char*>(resolve(src, S.B)) + (int64_t)n0 * ldbb);
+    if constexpr (NPART == 2) {
+      rA[1] = raw_rsrc(static_cast<const char*>(resolve(src, S.Alo)) + (int64_t)m0 * ldab);
+      rB[1] = raw_rsrc(static_cast<const char*>(resolve(src, S.Blo)) + (int64_t)n0 * ldbb);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      vA[c] = m0 + crow[c] < P.M ? (unsigned)(crow[c] * ldab + cq * 16) : OOB;
+      vB[c] = n0 + crow[c] < P.N ? (unsigned)(crow[c] * ldbb + cq * 16) : OOB;
+    }
+    Kb = S.K * EB;
+    kend = S.K;
+  };
+
+  int nsteps = 0;
+  for (int s = 0; s < nseg; ++s) nsteps += (P.seg[s].K + KSTEP - 1) / KSTEP;
+
+  ux4 ra[NPART][CH], rb[NPART][CH];
+  auto load = [&](int k0) {
+    const int kb0 = k0 * EB;
+    const bool kin = kb0 + cq * 16 < Kb;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const unsigned oa = kin ? vA[c] + kb0 : OOB;
+      const unsigned ob = kin ? vB[c] + kb0 : OOB;
+#pragma unroll
+      for (int p = 0; p < NPART; ++p) {
+        ra[p][c] = __builtin_amdgcn_raw_buffer_load_b128(rA[p], (int)oa, 0, 0);
+        rb[p][c] = __builtin_amdgcn_raw_buffer_load_b128(rB[p], (int)ob, 0, 0);
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NPART; ++p)
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        *reinterpret_cast<ux4*>(As[buf][p] + swz(crow[c], cq)) = ra[p][c];
+        *reinterpret_cast<ux4*>(Bs[buf][p] + swz(crow[c], cq)) = rb[p][c];
+      }
+  };
+
+  int s = 0, k0 = 0;
+  if (nsteps > 0) {
+    set_seg(0);
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    k0 += KSTEP;
+    while (k0 >= kend && s + 1 < nseg) {
+      ++s;
+      k0 = 0;
+      set_seg(s);
+    }
+    const bool has_next = step + 1 < nsteps;
+    if (has_next) load(k0);
+
+    ux4 fb[NPART][FN];
+#pragma unroll
+    for (int p = 0; p < NPART; ++p)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[p][j] = *reinterpret_cast<const ux4*>(Bs[buf][p] + boff + j * 16 * ROWB);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      ux4 fa[NPART];
+#pragma unroll
+      for (int p = 0; p < NPART; ++p) fa[p] = *reinterpret_cast<const ux4*>(As[buf][p] + aoff + i * 16 * ROWB);
+      if constexpr (MODE == M_BF16X3) {
+        const bfx8 ah = __builtin_bit_cast(bfx8, fa[0]);
+        const bfx8 al = __builtin_bit_cast(bfx8, fa[1]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const bfx8 bh = __builtin_bit_cast(bfx8, fb[0][j]);
+          const bfx8 bl = __builtin_bit_cast(bfx8, fb[1][j]);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah, acc[i][j], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int sub = 0; sub < 4; ++sub) {
+          const unsigned int ua = fa[0][sub];
+          const float a = __uint_as_float(ua);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            const unsigned int ub = fb[0][j][sub];
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ub), a, acc[i][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    if (has_next) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  store_tile<FM, FN>(acc, resolve(src, P.C), P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lg, P.alpha, P.beta);
+  }  // tiles
+}
+
 // ---------------------------------------------------------------------------
 // Pipelined variant for large problems: 256 x 128 output tile, 256 threads =
 // 4 waves (2 x 2) of 128 x 64, three LDS stages.  Operand slabs go global ->
@@ -391,14 +555,8 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big_kernel(Src src) {
 // ---------------------------------------------------------------------------
 
 constexpr int PBM = 256, PBN = 128, PNT = 256, NSTAGE = 3;
-constexpr unsigned OOB = 0x80000000u;
 
 typedef __attribute__((address_space(3))) void lds_void;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p) {
-  // raw (stride 0) buffer, num_records = 2^31 - 1 bytes, gfx9 data format word
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff, 0x00020000);
-}
 
 // s_waitcnt on vmcnt only (gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14])
 template <int N>
@@ -586,10 +744,22 @@ mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
     hipLaunchKernelGGL((gemm_mixed_pipe_kernel<MODE>), dim3(total), dim3(PNT), 0, stream, b);
   else if constexpr (VAR == V_BIG)
     hipLaunchKernelGGL((gemm_mixed_big_kernel<MODE, GemmBatch>), dim3(total), dim3(BNT), 0, stream, b);
+  else if constexpr (VAR == V_BIG2)
+    hipLaunchKernelGGL((gemm_mixed_big2_kernel<MODE, GemmBatch>), dim3(total), dim3(BNT), 0, stream, b);
   else
     hipLaunchKernelGGL((gemm_mixed_kernel<BM, BN, MODE, GemmBatch>), dim3(total), dim3(NT), 0, stream, b);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
+}
+
+// 256 rows of every operand lie within the 32-bit buffer offsets of gemm_mixed_big2_kernel
+template <int MODE>
+bool rows256_fit(const ProbSpec* probs, int nprob) {
+  constexpr int64_t EB = MODE == M_BF16X3 ? 2 : 4;
+  for (int i = 0; i < nprob; ++i)
+    for (int s = 0; s < probs[i].nseg; ++s)
+      if (256 * std::max(probs[i].seg[s].lda, probs[i].seg[s].ldb) * EB >= (1ll << 31)) return false;
+  return true;
 }
 
 template <int MODE>
@@ -613,7 +783,10 @@ mk_status dispatch(const ProbSpec* probs, int nprob, hipStream_t stream) {
   const int var = variant();
   // large tiles once they alone fill the 256 CUs; 128x128 while they fill them; else 64x64
   if (var == V_PIPE && tilesp >= 256 && off32) return launch_mixed<PBM, PBN, MODE, V_PIPE>(probs, nprob, stream);
-  if (var == V_BIG && tiles256 >= 256) return launch_mixed<256, 256, MODE, V_BIG>(probs, nprob, stream);
+  if (var == V_BIG2 && tiles256 >= 256 && rows256_fit<MODE>(probs, nprob))
+    return launch_mixed<256, 256, MODE, V_BIG2>(probs, nprob, stream);
+  if ((var == V_BIG || var == V_BIG2) && tiles256 >= 256)
+    return launch_mixed<256, 256, MODE, V_BIG>(probs, nprob, stream);
   if (tiles128 >= 256) return launch_mixed<128, 128, MODE, V_TILE>(probs, nprob, stream);
   return launch_mixed<64, 64, MODE, V_TILE>(probs, nprob, stream);
 }
@@ -622,7 +795,9 @@ template <int MODE>
 mk_status launch_group(const GemmGroup& g, int kind, int grid, hipStream_t stream) {
   if (g.total_tiles == 0) return MK_OK;
   ProfScope prof(MODE == M_BF16X3 ? "gemm_bf16x3" : "gemm_f32", stream);
-  if (kind == TK_256)
+  if (kind == TK_256 && variant() == V_BIG2 && g.rows256_fit)
+    hipLaunchKernelGGL((gemm_mixed_big2_kernel<MODE, GemmGroup>), dim3(grid), dim3(BNT), 0, stream, g);
+  else if (kind == TK_256)
     hipLaunchKernelGGL((gemm_mixed_big_kernel<MODE, GemmGroup>), dim3(grid), dim3(BNT), 0, stream, g);
   else if (kind == TK_128)
     hipLaunchKernelGGL((gemm_mixed_kernel<128, 128, MODE, GemmGroup>), dim3(grid), dim3(NT), 0, stream, g);

@@ -156,6 +156,11 @@ mk_status GroupPlan::run(const void* const base[NBASE], hipStream_t st) const {
   g.probs = static_cast<const GemmProb*>(dev.ptr);
   g.nprob = nprob;
   g.total_tiles = total_tiles;
+  const int64_t eb = prec == MK_FP32 ? 4 : 2;
+  g.rows256_fit = 1;
+  for (const ProbSpec& p : specs)
+    for (int s = 0; s < p.nseg; ++s)
+      if (256 * std::max(p.seg[s].lda, p.seg[s].ldb) * eb >= (1ll << 31)) g.rows256_fit = 0;
   for (int s = 0; s < NBASE; ++s) {
     g.base[s] = static_cast<const char*>(base ? base[s] : nullptr);
     MK_REQUIRE(reinterpret_cast<uintptr_t>(g.base[s]) % 16 == 0, MK_BAD_ARG,

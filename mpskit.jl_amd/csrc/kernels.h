@@ -90,4 +90,13 @@ mk_status k_split_bf16(const double* M, int64_t rows, int64_t cols, int64_t ld, 
 mk_status k_to_f32(const double* M, int64_t rows, int64_t cols, int64_t ld, int trans,
                    int64_t ldo, float* out, hipStream_t st);
 
+// One pass over x (d1 = d2 = 2, M = 2 Dl and N = 2 Dr multiples of 8) for the
+// mixed-precision nearest-neighbour apply: y = alpha CB(x) + beta y and the
+// K-contiguous conversions of x as left factor (xa, rows of length ka >= N) and
+// as right factor (xb, rows of length kb >= M); split = 1: bf16 hi/lo, 0: fp32
+// (xa_hi / xb_hi only).  Row padding is not written.
+mk_status k_nn_prologue(const double* P, const double* x, int64_t Dl, int64_t Dr, double alpha, double beta,
+                        double* y, int split, void* xa_hi, void* xa_lo, int64_t ka, void* xb_hi, void* xb_lo,
+                        int64_t kb, hipStream_t st);
+
 }  // namespace mk

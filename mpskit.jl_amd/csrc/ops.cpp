@@ -257,11 +257,18 @@ mk_status apply_blocks(Op* op, const double* x, double* y, double alpha, double 
   double* out = alpha != 1.0 ? static_cast<double*>(tmp) : y;
   const double b0 = alpha != 1.0 ? 0.0 : beta;
   // every element: physical term (or the beta scaling); then the blocks accumulate
-  if (op->hasP)
+  const bool fused = prec != MK_FP64 && op->hasP && d1 == 2 && d2 == 2 && M % 8 == 0 && N % 8 == 0;
+  if (fused) {
+    const MixedView xa = view_layout(M, N, 'A', prec, mv);
+    const MixedView xb = view_layout(M, N, 'B', prec, static_cast<char*>(mv) + va);
+    MK_TRY(k_nn_prologue(static_cast<const double*>(op->P.ptr), x, Dl, Dr, 1.0, b0, out, prec == MK_BF16X3, xa.hi,
+                         xa.lo, xa.kpad, xb.hi, xb.lo, xb.kpad, st));
+  } else if (op->hasP) {
     MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, 1.0, b0, out, st));
-  else
+  } else {
     MK_TRY(k_axpby(0.0, out, b0, out, n, st));
-  if (prec != MK_FP64) {
+  }
+  if (prec != MK_FP64 && !fused) {
     MixedView xa, xb;
     MK_TRY(convert_view(ctx, x, M, N, M, 'A', prec, &xa, mv));
     MK_TRY(convert_view(ctx, x, M, N, M, 'B', prec, &xb, static_cast<char*>(mv) + va));
@@ -290,6 +297,10 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
   BetaSeq bs{beta};
   // mixed-precision views of x: as left factor (M x N) and as right factor (K = M rows)
   MixedView xa, xb, xca;
+  // nearest-neighbour chains with d = 2: the CB term and both conversions of x
+  // come from one pass over x
+  const bool fused = prec != MK_FP64 && op->hasP && op->IC.present() && op->BE.present() && !op->CA.present() &&
+                     op->AB.empty() && d1 == 2 && d2 == 2 && M % 8 == 0 && N % 8 == 0;
   if (prec != MK_FP64) {
     void *ma = nullptr, *mb = nullptr, *mc = nullptr;
     const bool need_a = op->IC.present(), need_b = op->BE.present() || !op->AB.empty();
@@ -297,13 +308,21 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
     MK_TRY(carve(ctx, {{need_a ? view_bytes(M, N, 'A', prec) : 0, &ma},
                        {need_b ? view_bytes(M, N, 'B', prec) : 0, &mb},
                        {need_ca ? view_bytes(Dl, d1 * N, 'A', prec) : 0, &mc}}));
-    if (need_a) MK_TRY(convert_view(ctx, x, M, N, M, 'A', prec, &xa, ma));
-    if (need_b) MK_TRY(convert_view(ctx, x, M, N, M, 'B', prec, &xb, mb));
-    if (need_ca) MK_TRY(convert_view(ctx, x, Dl, d1 * N, Dl, 'A', prec, &xca, mc));
+    if (fused) {
+      xa = view_layout(M, N, 'A', prec, ma);
+      xb = view_layout(M, N, 'B', prec, mb);
+    } else {
+      if (need_a) MK_TRY(convert_view(ctx, x, M, N, M, 'A', prec, &xa, ma));
+      if (need_b) MK_TRY(convert_view(ctx, x, M, N, M, 'B', prec, &xb, mb));
+      if (need_ca) MK_TRY(convert_view(ctx, x, Dl, d1 * N, Dl, 'A', prec, &xca, mc));
+    }
   }
   const GemmShape nn{};
   // two-site CB term first: a memory-bound pass that reads x once
-  if (op->hasP)
+  if (fused)
+    MK_TRY(k_nn_prologue(static_cast<const double*>(op->P.ptr), x, Dl, Dr, alpha, bs.next(), y,
+                         prec == MK_BF16X3, xa.hi, xa.lo, xa.kpad, xb.hi, xb.lo, xb.kpad, ctx->stream));
+  else if (op->hasP)
     MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, alpha, bs.next(), y,
                    ctx->stream));
   if (op->IC.present() || op->BE.present()) {

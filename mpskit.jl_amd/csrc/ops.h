@@ -72,6 +72,8 @@ struct MixedView {
 
 mk_status convert_view(Ctx* ctx, const double* M, int64_t rows, int64_t cols, int64_t ld, char role,
                        int prec, MixedView* out, void* mem);
+// Where convert_view puts the converted (rows x cols) view in `mem`, without converting.
+MixedView view_layout(int64_t rows, int64_t cols, char role, int prec, void* mem);
 size_t view_bytes(int64_t rows, int64_t cols, char role, int prec);
 mk_status run_gemm(const ProbSpec* p, int np, int prec, GemmShape sh, hipStream_t st);
 
