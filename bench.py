@@ -81,6 +81,35 @@ def time_applies(op, x, y, prec, steps, warmup):
     return start.elapsed_time(stop) / steps
 
 
+def vendor_gemm_reference(D, d=2, reps=20):
+    """torch.matmul (hipBLASLt / rocBLAS) on the GEMM shape of one apply: the
+    two-segment product y = x IC + BE x is one M = N = dD, K = 2 dD GEMM.
+    bf16 is timed once and reported x3 (the three products of the split
+    scheme); float64 once.  Context for the roofline fractions, not a path of
+    the library."""
+    import torch
+    n = d * D
+    out = {}
+    for name, dt in (("bf16", torch.bfloat16), ("fp64", torch.float64)):
+        A = torch.randn(n, 2 * n, dtype=dt, device="cuda")
+        B = torch.randn(2 * n, n, dtype=dt, device="cuda")
+        for _ in range(3):
+            C = A @ B
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            C = A @ B
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = dict(ms=ms, tflops=2.0 * n * n * 2 * n / (ms * 1e-3) / 1e12)
+        del A, B, C
+    out["bf16_x3_ms"] = 3 * out["bf16"]["ms"]
+    out["shape"] = f"M=N={n}, K={2 * n}"
+    return out
+
+
 def cpu_baseline(D, budget_s=20.0):
     """float64 oracle restatement (numpy/OpenBLAS) of the same apply on host cores."""
     from threadpoolctl import threadpool_info, threadpool_limits
@@ -419,6 +448,8 @@ def main():
                           "traffic": traffic_from_profile("gemm_f64_kernel<128,128>", D),
                           "kernel": "gemm_f64_kernel<128,128>"},
     }
+    print("# vendor GEMM reference ...", file=sys.stderr, flush=True)
+    out["vendor_gemm_reference"] = vendor_gemm_reference(D)
     if not args.no_sweep:
         sw = {}
         for prec in ("fp64", "bf16x3"):

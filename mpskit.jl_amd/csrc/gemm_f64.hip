@@ -1,12 +1,13 @@
 // FP64 MFMA GEMM for CDNA4 (gfx950): v_mfma_f64_16x16x4_f64, LDS-staged
-// BK=32 tiles, double-buffered through registers, 256 threads = 4 waves of 64
-// lanes in a 2x2 arrangement.  One workgroup computes one BM x BN output tile
+// BK = 16 (or 32) deep tiles, double-buffered through registers, 256 threads =
+// 4 waves of 64 lanes in a 2x2 arrangement.  One workgroup computes one BM x BN output tile
 // of one problem, summing every segment of that problem into the same
 // accumulators before a single fused epilogue (alpha/beta and optional CB term).
 #include "gemm_dev.h"
 #include "prof.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace mk {
 namespace {
@@ -15,12 +16,12 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 typedef double dx2 __attribute__((ext_vector_type(2)));
 
 constexpr int NT = 256;
-constexpr int BK = 32;
+constexpr int BK32 = 32;  // default K depth of a stage
 constexpr int PAD = 16;     // LDS row padding (doubles): alternates 128-B bank halves
 
 // Stage a tile whose contiguous memory direction is the LDS fast direction.
 // element (r, k) lives at p[(r0 + r) + (k0 + k) * ld]; LDS tile S[k][r].
-template <int R, int LDR>
+template <int R, int LDR, int BK>
 __device__ __forceinline__ void load_fastdir(const double* __restrict__ p, int64_t ld, int r0,
                                              int k0, int Rlim, int Klim, bool vec,
                                              dx2 (&reg)[R * BK / 2 / NT]) {
@@ -45,7 +46,7 @@ __device__ __forceinline__ void load_fastdir(const double* __restrict__ p, int64
   }
 }
 
-template <int R, int LDR>
+template <int R, int LDR, int BK>
 __device__ __forceinline__ void store_fastdir(double (*S)[LDR], const dx2 (&reg)[R * BK / 2 / NT]) {
   constexpr int CH = R * BK / 2 / NT;
 #pragma unroll
@@ -65,7 +66,7 @@ __device__ __forceinline__ void store_fastdir(double (*S)[LDR], const dx2 (&reg)
 // aligned 32-column group, undone by the fragment reads).
 __device__ __forceinline__ int kswz(int k) { return k & 30; }
 
-template <int R, int LDR>
+template <int R, int LDR, int BK>
 __device__ __forceinline__ void load_kdir(const double* __restrict__ p, int64_t ld, int r0, int k0,
                                           int Rlim, int Klim, bool vec,
                                           dx2 (&reg)[R * BK / 2 / NT]) {
@@ -90,7 +91,7 @@ __device__ __forceinline__ void load_kdir(const double* __restrict__ p, int64_t 
   }
 }
 
-template <int R, int LDR>
+template <int R, int LDR, int BK>
 __device__ __forceinline__ void store_kdir(double (*S)[LDR], const dx2 (&reg)[R * BK / 2 / NT]) {
   constexpr int CH = R * BK / 2 / NT;
 #pragma unroll
@@ -103,8 +104,9 @@ __device__ __forceinline__ void store_kdir(double (*S)[LDR], const dx2 (&reg)[R 
   }
 }
 
-template <int BM, int BN, bool TA, bool TB, class Src>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(Src src) {
+template <int BM, int BN, bool TA, bool TB, class Src, int BK = BK32>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BK == 16 ? 2 : 1)))
+void gemm_f64_kernel(Src src) {
   constexpr int LDA_S = BM + PAD;
   constexpr int LDB_S = BN + PAD;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -144,16 +146,16 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Src src) {
     const bool vec = S.vec != 0;
     const double* A = static_cast<const double*>(resolve(src, S.A));
     const double* B = static_cast<const double*>(resolve(src, S.B));
-    if (!TA) load_fastdir<BM, LDA_S>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
-    else     load_kdir<BM, LDA_S>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
-    if (!TB) load_kdir<BN, LDB_S>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
-    else     load_fastdir<BN, LDB_S>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
+    if (!TA) load_fastdir<BM, LDA_S, BK>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
+    else     load_kdir<BM, LDA_S, BK>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
+    if (!TB) load_kdir<BN, LDB_S, BK>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
+    else     load_fastdir<BN, LDB_S, BK>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
   };
   auto store = [&](int buf) {
-    if (!TA) store_fastdir<BM, LDA_S>(As[buf], ra);
-    else     store_kdir<BM, LDA_S>(As[buf], ra);
-    if (!TB) store_kdir<BN, LDB_S>(Bs[buf], rb);
-    else     store_fastdir<BN, LDB_S>(Bs[buf], rb);
+    if (!TA) store_fastdir<BM, LDA_S, BK>(As[buf], ra);
+    else     store_kdir<BM, LDA_S, BK>(As[buf], ra);
+    if (!TB) store_kdir<BN, LDB_S, BK>(Bs[buf], rb);
+    else     store_fastdir<BN, LDB_S, BK>(Bs[buf], rb);
   };
 
   int s = 0, k0 = 0;
@@ -217,6 +219,26 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Src src) {
   }  // tiles
 }
 
+// K depth of a stage, read once from MK_F64_BK (16 or 32; default 16).  BK = 16
+// halves the LDS footprint of a 128 x 128 workgroup (74 KB instead of 147 KB),
+// so two workgroups (two waves per SIMD) share a CU: 24% faster at
+// M = N = 4096, K = 8192 than BK = 32 with one wave per SIMD.
+int f64_bk() {
+  static const int bk = [] {
+    const char* e = getenv("MK_F64_BK");
+    return (e && atoi(e) == 32) ? 32 : 16;
+  }();
+  return bk;
+}
+
+template <int BM, int BN, class Src, int BK>
+void launch4(GemmShape sh, dim3 grid, dim3 block, hipStream_t stream, const Src& b) {
+  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, Src, BK>), grid, block, 0, stream, b);
+  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, Src, BK>), grid, block, 0, stream, b);
+  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, Src, BK>), grid, block, 0, stream, b);
+  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, Src, BK>), grid, block, 0, stream, b);
+}
+
 template <int BM, int BN>
 mk_status launch_cfg(const ProbSpec* probs, int nprob, GemmShape sh, hipStream_t stream) {
   GemmBatch b{};
@@ -254,10 +276,8 @@ mk_status launch_cfg(const ProbSpec* probs, int nprob, GemmShape sh, hipStream_t
   if (total == 0) return MK_OK;
   dim3 grid(total), block(NT);
   ProfScope prof("gemm_f64", stream);
-  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, GemmBatch>), grid, block, 0, stream, b);
-  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, GemmBatch>), grid, block, 0, stream, b);
-  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, GemmBatch>), grid, block, 0, stream, b);
-  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, GemmBatch>), grid, block, 0, stream, b);
+  if (f64_bk() == 16) launch4<BM, BN, GemmBatch, 16>(sh, grid, block, stream, b);
+  else launch4<BM, BN, GemmBatch, BK32>(sh, grid, block, stream, b);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
 }
@@ -267,10 +287,8 @@ mk_status launch_group(const GemmGroup& g, GemmShape sh, int grid, hipStream_t s
   if (g.total_tiles == 0) return MK_OK;
   dim3 gr(grid), block(NT);
   ProfScope prof("gemm_f64", stream);
-  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, GemmGroup>), gr, block, 0, stream, g);
-  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, GemmGroup>), gr, block, 0, stream, g);
-  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, GemmGroup>), gr, block, 0, stream, g);
-  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, GemmGroup>), gr, block, 0, stream, g);
+  if (f64_bk() == 16) launch4<BM, BN, GemmGroup, 16>(sh, gr, block, stream, g);
+  else launch4<BM, BN, GemmGroup, BK32>(sh, gr, block, stream, g);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
 }
@@ -279,9 +297,12 @@ template <int BM, int BN>
 int occupancy() {
   static const int occ = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64_kernel<BM, BN, false, false, GemmGroup>, NT, 0) !=
-        hipSuccess)
-      n = 1;
+    const hipError_t e =
+        f64_bk() == 16
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64_kernel<BM, BN, false, false, GemmGroup, 16>, NT, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64_kernel<BM, BN, false, false, GemmGroup, BK32>, NT,
+                                                           0);
+    if (e != hipSuccess) n = 1;
     return n > 0 ? n : 1;
   }();
   return occ;
