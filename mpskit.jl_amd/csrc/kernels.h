@@ -66,6 +66,15 @@ mk_status k_regularize(double* v, const double* lvec, const double* rvec, int64_
 // out[j] = <V_j, w> for j < k  (V column-major n x k, leading dim ldv)
 mk_status k_multidot(const double* V, int64_t ldv, int k, const double* w, int64_t n,
                      double* out, double* partial, hipStream_t st);
+// Fused classical Gram-Schmidt passes over the k <= 32 columns of V (ldv):
+// u <- u - V cin, then out[0..k) = V^T u (update_dot) or out[0] = u^T u,
+// out[1] = w^T u (update_norm), all in one read of V.  partial holds
+// cgs_blocks(n) * k doubles.
+int cgs_blocks(int64_t n);
+mk_status k_cgs_update_dot(const double* V, int64_t ldv, int k, const double* cin, double* u, int64_t n,
+                           double* out, double* partial, hipStream_t st);
+mk_status k_cgs_update_norm(const double* V, int64_t ldv, int k, const double* cin, double* u, const double* w,
+                            int64_t n, double* out, double* partial, hipStream_t st);
 // w -= V c (c on device, length k)
 mk_status k_multiaxpy(const double* V, int64_t ldv, int k, const double* c, double* w,
                       int64_t n, hipStream_t st);

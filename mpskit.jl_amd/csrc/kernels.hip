@@ -368,6 +368,49 @@ __global__ void reduce_rows_kernel(const double* __restrict__ partial, int nb, d
   if (threadIdx.x == 0) out[j] = s;
 }
 
+// Fused classical Gram-Schmidt update: u <- u - V c (c = cin[0..k)) and, with
+// the updated u, the partial sums of V^T u (MODE 0, k outputs) or of u^T u and
+// w^T u (MODE 1, 2 outputs).  Each thread keeps its row of V in registers
+// between the update and the dot products, so V is read once for both.
+// k <= CGS_KMAX; partial sums go to partial[o * gridDim.x + blockIdx.x].
+constexpr int CGS_KMAX = 32;
+template <int MODE>
+__global__ __launch_bounds__(TPB) void cgs_update_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                                                         const double* __restrict__ cin, double* __restrict__ u,
+                                                         const double* __restrict__ w, int64_t n,
+                                                         double* __restrict__ partial) {
+  __shared__ double sh[TPB / 64];
+  constexpr int NO = MODE == 0 ? CGS_KMAX : 2;
+  double c[CGS_KMAX], acc[NO];
+#pragma unroll
+  for (int j = 0; j < CGS_KMAX; ++j) c[j] = j < k ? cin[j] : 0.0;
+#pragma unroll
+  for (int o = 0; o < NO; ++o) acc[o] = 0.0;
+  GRID_STRIDE(r, n) {
+    double q[CGS_KMAX];
+#pragma unroll
+    for (int j = 0; j < CGS_KMAX; ++j) q[j] = j < k ? V[r + (int64_t)j * ldv] : 0.0;
+    double v = u[r];
+#pragma unroll
+    for (int j = 0; j < CGS_KMAX; ++j) v -= c[j] * q[j];
+    u[r] = v;
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int j = 0; j < CGS_KMAX; ++j) acc[j] += q[j] * v;
+    } else {
+      acc[0] += v * v;
+      acc[1] += w[r] * v;
+    }
+  }
+  const int nout = MODE == 0 ? k : 2;
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    if (o >= nout) break;
+    const double t = block_sum(acc[o], sh);
+    if (threadIdx.x == 0) partial[(int64_t)o * gridDim.x + blockIdx.x] = t;
+  }
+}
+
 __global__ void multiaxpy_kernel(const double* __restrict__ V, int64_t ldv, int k,
                                  const double* __restrict__ c, double* __restrict__ w, int64_t n) {
   GRID_STRIDE(i, n) {
@@ -661,6 +704,30 @@ mk_status k_multidot(const double* V, int64_t ldv, int k, const double* w, int64
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
 }
+int cgs_blocks(int64_t n) { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(n, (int64_t)TPB * 4))); }
+
+mk_status k_cgs_update_dot(const double* V, int64_t ldv, int k, const double* cin, double* u, int64_t n,
+                           double* out, double* partial, hipStream_t st) {
+  MK_REQUIRE(k >= 1 && k <= CGS_KMAX, MK_BAD_ARG, "cgs_update_dot: k = %d not in [1, %d]", k, CGS_KMAX);
+  const int nb = cgs_blocks(n);
+  hipLaunchKernelGGL(cgs_update_kernel<0>, dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, nullptr, n, partial);
+  MK_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(k), dim3(TPB), 0, st, partial, nb, out);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+mk_status k_cgs_update_norm(const double* V, int64_t ldv, int k, const double* cin, double* u, const double* w,
+                            int64_t n, double* out, double* partial, hipStream_t st) {
+  MK_REQUIRE(k >= 1 && k <= CGS_KMAX, MK_BAD_ARG, "cgs_update_norm: k = %d not in [1, %d]", k, CGS_KMAX);
+  const int nb = cgs_blocks(n);
+  hipLaunchKernelGGL(cgs_update_kernel<1>, dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, w, n, partial);
+  MK_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(2), dim3(TPB), 0, st, partial, nb, out);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
 mk_status k_multiaxpy(const double* V, int64_t ldv, int k, const double* c, double* w, int64_t n,
                       hipStream_t st) {
   if (k <= 0) return MK_OK;

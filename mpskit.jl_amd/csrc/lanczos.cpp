@@ -142,9 +142,11 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
   MK_TRY(Qb.alloc((size_t)n * (m + 2) * 8));
   MK_TRY(wb.alloc((size_t)n * 8));
   MK_TRY(ub.alloc((size_t)n * 8));
-  MK_TRY(cb.alloc((size_t)(m + 4) * 8));
+  // coefficients: generic dots at [0, m + 4); fused passes at [FC, FC + 2 * 32 + 2)
+  constexpr int FC = 1024;
+  MK_TRY(cb.alloc((size_t)(FC + 2 * 32 + 2) * 8));
   const int64_t nb = std::max<int64_t>(1, cdiv(n, 256 * 16));
-  MK_TRY(pb.alloc((size_t)nb * (m + 4) * 8));
+  MK_TRY(pb.alloc((size_t)std::max<int64_t>(nb * (m + 4), (int64_t)cgs_blocks(n) * 34) * 8));
   double* Q = static_cast<double*>(Qb.ptr);
   double* w = static_cast<double*>(wb.ptr);
   double* u = static_cast<double*>(ub.ptr);
@@ -201,14 +203,34 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
       MK_TRY(op_apply(op, col(j + 1), u, 1.0, 0.0, precision));
       ++ops;
       // two classical Gram-Schmidt passes against [x, v_0..v_j]
-      MK_TRY(dv.dots(Q, n, j + 2, u, c));
-      MK_TRY(dv.subtract(Q, n, j + 2, u));
-      MK_TRY(dv.dots(Q, n, j + 2, u, c2));
-      MK_TRY(dv.subtract(Q, n, j + 2, u));
+      double beta = 0.0, uw = 0.0;
+      const int kq = j + 2;
+      if (kq <= 32) {
+        // fused: c1 = Q^T u; u -= Q c1 with c2 = Q^T u in the same read of Q;
+        // u -= Q c2 with |u|^2 and <w, u>; one readback for the step
+        double* cd = dv.coef + FC;
+        MK_TRY(k_multidot(Q, n, kq, u, n, cd, dv.partial, st));
+        MK_TRY(k_cgs_update_dot(Q, n, kq, cd, u, n, cd + 32, dv.partial, st));
+        MK_TRY(k_cgs_update_norm(Q, n, kq, cd + 32, u, w, n, cd + 64, dv.partial, st));
+        std::vector<double> h(2 * 32 + 2);
+        MK_CHECK_HIP(hipMemcpyAsync(h.data(), cd, h.size() * 8, hipMemcpyDeviceToHost, st));
+        MK_CHECK_HIP(hipStreamSynchronize(st));
+        c.assign(h.begin(), h.begin() + kq);
+        c2.assign(h.begin() + 32, h.begin() + 32 + kq);
+        beta = std::sqrt(std::max(h[64], 0.0));
+        uw = h[65];
+      } else {
+        MK_TRY(dv.dots(Q, n, kq, u, c));
+        MK_TRY(dv.subtract(Q, n, kq, u));
+        MK_TRY(dv.dots(Q, n, kq, u, c2));
+        MK_TRY(dv.subtract(Q, n, kq, u));
+        MK_TRY(dv.norm(u, n, beta));
+        std::vector<double> t;
+        MK_TRY(dv.dots(w, n, 1, u, t));
+        uw = t[0];
+      }
       // projected matrix among the Krylov vectors; the x-row stays float64 (<v_i, w>)
       for (int i = 1; i < j + 2; ++i) G[(size_t)i * ldg + (j + 1)] = c[i] + c2[i];
-      double beta = 0.0;
-      MK_TRY(dv.norm(u, n, beta));
       MK_REQUIRE(std::isfinite(beta), MK_NONFINITE, "eigsolve: non-finite Krylov vector");
       double lam;
       lowest(G, ldg, j + 2, lam, s);
@@ -217,9 +239,8 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
       if (est_last <= thresh || beta <= 1e-14 * std::fabs(theta) || j + 1 == m) break;
       MK_TRY(k_axpby(1.0 / beta, u, 0.0, col(j + 2), n, st));
       G[(size_t)(j + 2) * ldg + (j + 1)] = beta;
-      std::vector<double> xw;
-      MK_TRY(dv.dots(col(j + 2), n, 1, w, xw));
-      G[(size_t)(j + 2)] = G[(size_t)(j + 2) * ldg] = xw[0];
+      // <v_{j+1}, w> = <u, w> / beta
+      G[(size_t)(j + 2)] = G[(size_t)(j + 2) * ldg] = uw / beta;
     }
     // lowest Ritz vector of span{x, v_0..v_{k-1}} becomes the new x
     double lam;
