@@ -37,6 +37,9 @@ struct PrecPlan {
   GroupPlan gemm;
   DevBuf split;
   int nsplit = 0, split_max = 0;
+  // nearest-neighbour d = 2 batches: CB term and both conversions in one pass
+  DevBuf nn;
+  int nnn = 0, nn_max = 0;
   int64_t scratch_bytes = 0;
 };
 
@@ -103,6 +106,8 @@ mk_status Batch::ensure_plan(int prec) {
   const bool mixed = prec != MK_FP64;
   std::vector<ProbSpec> probs(n);
   std::vector<SplitItem> items;
+  std::vector<NNItem> nn_items;
+  bool all_nn = mixed;
   std::vector<ProbSpec> block_specs;
   int64_t sbytes = 0;
   for (int i = 0; i < n; ++i) {
@@ -148,6 +153,27 @@ mk_status Batch::ensure_plan(int prec) {
       p.add(bf ? m.hi.ptr : m.f32.ptr, m.kpad, rel_ptr(3, b_hi), ldb, M, bf ? m.lo.ptr : nullptr,
             bf ? rel_ptr(3, b_lo) : nullptr);
     }
+    if (op->hasP && op->IC.present() && op->BE.present() && op->blocks.empty() && op->d1 == 2 && op->d2 == 2 &&
+        M % 8 == 0 && N % 8 == 0) {
+      NNItem it{};
+      it.P = static_cast<const double*>(op->P.ptr);
+      it.x = static_cast<const double*>(x);
+      it.y = const_cast<double*>(static_cast<const double*>(rel_ptr(2, xoff(i))));
+      it.xa_hi = const_cast<void*>(rel_ptr(3, a_hi));
+      it.xa_lo = bf ? const_cast<void*>(rel_ptr(3, a_lo)) : nullptr;
+      it.xb_hi = const_cast<void*>(rel_ptr(3, b_hi));
+      it.xb_lo = bf ? const_cast<void*>(rel_ptr(3, b_lo)) : nullptr;
+      it.Dl = op->Dl;
+      it.Dr = op->Dr;
+      it.ka = lda;
+      it.kb = ldb;
+      nn_tiles(op->Dl, op->Dr, it.tiles_a, it.tiles);
+      it.alpha = 1.0;
+      it.beta = 0.0;
+      nn_items.push_back(it);
+    } else {
+      all_nn = false;
+    }
     if (!op->blocks.empty()) block_probs(op, prec, xoff(i), a_hi, b_hi, 1.0, block_specs);
   }
   // operators with charge blocks contribute their block problems instead of the dense one
@@ -157,6 +183,14 @@ mk_status Batch::ensure_plan(int prec) {
       if (ops[i]->blocks.empty()) kept.push_back(probs[i]);
     kept.insert(kept.end(), block_specs.begin(), block_specs.end());
     probs.swap(kept);
+  }
+  if (all_nn && !nn_items.empty() && nn_items.size() == (size_t)n) {
+    // the fused pass replaces the physical pass and both conversions
+    MK_TRY(pp.nn.alloc(nn_items.size() * sizeof(NNItem)));
+    MK_CHECK_HIP(hipMemcpy(pp.nn.ptr, nn_items.data(), nn_items.size() * sizeof(NNItem), hipMemcpyHostToDevice));
+    pp.nnn = (int)nn_items.size();
+    for (const NNItem& it : nn_items) pp.nn_max = std::max(pp.nn_max, it.tiles);
+    items.clear();
   }
   if (!items.empty()) {
     MK_TRY(pp.split.alloc(items.size() * sizeof(SplitItem)));
@@ -188,6 +222,10 @@ mk_status Batch::apply(const double* x, double* y, int prec, hipStream_t st) {
   }
   const PrecPlan& pp = plan[prec];
   const void* base[NBASE] = {nullptr, x, y, scratch.ptr};
+  if (pp.nnn) {
+    MK_TRY(nn_group(static_cast<const NNItem*>(pp.nn.ptr), pp.nnn, pp.nn_max, prec == MK_BF16X3, base, st));
+    return pp.gemm.run(base, st);
+  }
   if (nphys) MK_TRY(phys2_group(static_cast<const PhysItem*>(phys.ptr), nphys, phys_max, d12, base, st));
   if (pp.nsplit)
     MK_TRY(split_group(static_cast<const SplitItem*>(pp.split.ptr), pp.nsplit, pp.split_max, prec == MK_BF16X3, base,

@@ -1,5 +1,6 @@
 // Memory-bound helper kernels; see kernels.h for the contracts.
 #include "kernels.h"
+#include "convert_dev.h"
 
 #include <hip/hip_bf16.h>
 
@@ -454,13 +455,6 @@ __global__ void nonfinite_kernel(const double* __restrict__ x, int64_t n, int* f
 
 // ---- precision conversion ---------------------------------------------------
 
-__device__ __forceinline__ void split2(double v, __hip_bfloat16& hi, __hip_bfloat16& lo) {
-  const float f = (float)v;
-  hi = __float2bfloat16(f);
-  const double r = v - (double)__bfloat162float(hi);
-  lo = __float2bfloat16((float)r);
-}
-
 // plain conversion (no transposition): out[c*ldo + r] = M[r + c*ld], zero for rows <= r < ldo
 __global__ void split_plain_kernel(const double* __restrict__ M, int64_t rows, int64_t cols, int64_t ld,
                                    int64_t ldo, __hip_bfloat16* __restrict__ hi,
@@ -523,20 +517,6 @@ __global__ void f32_trans_kernel(const double* __restrict__ M, int64_t rows, int
   }
 }
 
-// One pass over x for the mixed-precision nearest-neighbour two-site apply
-// (d1 = d2 = 2, x an M x N column-major matrix, M = 2 Dl, N = 2 Dr, row
-// r' = a + Dl h, column c' = c + Dr j):
-//   y = alpha * CB(x) + beta * y                       (the physical term)
-//   xb[c' * kb + r'] = conv(x[r', c'])                 (x as right factor, K = rows)
-//   xa[r' * ka + c'] = conv(x[r', c'])                 (x as left factor, K = cols)
-// with conv the bf16 hi/lo split (SPLIT) or the fp32 rounding.  A block owns an
-// (a, c) tile of 32 x 64 and all four (h, j) sub-tiles of it, so x is read
-// once: the thread holding x[a, :, c, :] produces the CB outputs of that pair,
-// and the converted values pass through LDS (33 KB) for the transposed xa
-// stores.  Row padding of xa / xb is not written (K = N and M are multiples of
-// 8 here, so no K chunk the GEMMs read reaches into it).
-constexpr int PRO_A = 32, PRO_C = 64;
-
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void nn_prologue_kernel(const double* __restrict__ P,
                                                           const double* __restrict__ x, int64_t Dl, int64_t Dr,
@@ -544,76 +524,9 @@ __global__ __launch_bounds__(256) void nn_prologue_kernel(const double* __restri
                                                           void* __restrict__ xa_hi, void* __restrict__ xa_lo,
                                                           int64_t ka, void* __restrict__ xb_hi,
                                                           void* __restrict__ xb_lo, int64_t kb) {
-  __shared__ unsigned int t[4][PRO_C][PRO_A + 1];   // converted sub-tile (h, j), [c][a]
-  const int64_t a0 = (int64_t)blockIdx.x * PRO_A, c0 = (int64_t)blockIdx.y * PRO_C;
-  const int64_t M = 2 * Dl;
-  double p[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) p[i] = P[i];
-  {
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // a offset, c phase (8)
-    const int64_t a = a0 + tx;
-#pragma unroll
-    for (int k = 0; k < PRO_C / 8; ++k) {
-      const int cc = ty + 8 * k;
-      const int64_t c = c0 + cc;
-      if (a < Dl && c < Dr) {
-        double xv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) xv[q] = x[a + Dl * (q & 1) + M * (c + Dr * (q >> 1))];
-#pragma unroll
-        for (int o = 0; o < 4; ++o) {
-          double s = 0.0;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) s += p[(o & 1) + 2 * ((o >> 1) + 2 * q)] * xv[q];
-          double* yp = y + a + Dl * (o & 1) + M * (c + Dr * (o >> 1));
-          double v = alpha * s;
-          if (beta != 0.0) v += beta * *yp;
-          *yp = v;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int64_t rr = a + Dl * (q & 1), cl = c + Dr * (q >> 1);
-          unsigned int packed;
-          if constexpr (SPLIT) {
-            __hip_bfloat16 h, l;
-            split2(xv[q], h, l);
-            static_cast<__hip_bfloat16*>(xb_hi)[cl * kb + rr] = h;
-            static_cast<__hip_bfloat16*>(xb_lo)[cl * kb + rr] = l;
-            packed = (unsigned int)__builtin_bit_cast(unsigned short, h) |
-                     ((unsigned int)__builtin_bit_cast(unsigned short, l) << 16);
-          } else {
-            const float f = (float)xv[q];
-            static_cast<float*>(xb_hi)[cl * kb + rr] = f;
-            packed = __float_as_uint(f);
-          }
-          t[q][cc][tx] = packed;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  {
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // c offset, a phase (4)
-    const int64_t c = c0 + tx;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-#pragma unroll
-      for (int k = 0; k < PRO_A / 4; ++k) {
-        const int aa = ty + 4 * k;
-        const int64_t ar = a0 + aa;
-        if (ar >= Dl || c >= Dr) continue;
-        const unsigned int v = t[q][tx][aa];
-        const int64_t idx = (ar + Dl * (q & 1)) * ka + c + Dr * (q >> 1);
-        if constexpr (SPLIT) {
-          static_cast<unsigned short*>(xa_hi)[idx] = (unsigned short)(v & 0xffffu);
-          static_cast<unsigned short*>(xa_lo)[idx] = (unsigned short)(v >> 16);
-        } else {
-          static_cast<unsigned int*>(xa_hi)[idx] = v;
-        }
-      }
-    }
-  }
+  __shared__ ProTile t;
+  nn_prologue_tile<SPLIT>(P, x, Dl, Dr, alpha, beta, y, xa_hi, xa_lo, ka, xb_hi, xb_lo, kb,
+                          (int64_t)blockIdx.x * PRO_A, (int64_t)blockIdx.y * PRO_C, t);
 }
 
 }  // namespace

@@ -59,4 +59,24 @@ mk_status phys2_group(const PhysItem* items, int n, int max_blocks, int d12, con
                       hipStream_t st);
 int phys2_blocks(int64_t Dl, int64_t Dr);
 
+// One nearest-neighbour two-site prologue (d1 = d2 = 2): y = alpha CB(x) + beta y
+// and the K-contiguous conversions of x as left factor (xa, row length ka) and
+// as right factor (xb, row length kb); x, y, xa, xb may be relative, P absolute.
+struct NNItem {
+  const double* P;
+  const double* x;
+  double* y;
+  void* xa_hi;
+  void* xa_lo;
+  void* xb_hi;
+  void* xb_lo;
+  int64_t Dl, Dr, ka, kb;
+  int32_t tiles_a, tiles;
+  double alpha, beta;
+};
+// Items along blockIdx.y, one (32 x 64) (a, c) tile per block; max_tiles = max tiles.
+mk_status nn_group(const NNItem* items, int n, int max_tiles, bool bf16, const void* const base[NBASE],
+                   hipStream_t st);
+void nn_tiles(int64_t Dl, int64_t Dr, int32_t& tiles_a, int32_t& tiles);
+
 }  // namespace mk

@@ -11,6 +11,7 @@
 
 #include "gemm.h"
 #include "seg.h"
+#include "convert_dev.h"
 
 namespace mk {
 namespace {
@@ -132,13 +133,6 @@ __global__ void seg_normalize_kernel(const double* x, const double* __restrict__
 
 // ---- grouped conversions ------------------------------------------------------
 
-__device__ __forceinline__ void split2(double v, __hip_bfloat16& hi, __hip_bfloat16& lo) {
-  const float f = (float)v;
-  hi = __float2bfloat16(f);
-  const double r = v - (double)__bfloat162float(hi);
-  lo = __float2bfloat16((float)r);
-}
-
 template <bool BF>
 __device__ __forceinline__ void put(const SplitItem& d, void* hi, void* lo, int64_t idx, double v) {
   if (BF) {
@@ -215,6 +209,17 @@ __global__ void phys2_group_kernel(const PhysItem* __restrict__ items, BaseArgs 
     if (d.beta != 0.0) v += d.beta * *yp;
     *yp = v;
   }
+}
+
+template <bool BF>
+__global__ __launch_bounds__(256) void nn_group_kernel(const NNItem* __restrict__ items, BaseArgs ba) {
+  __shared__ ProTile t;
+  const NNItem d = items[blockIdx.y];
+  if ((int)blockIdx.x >= d.tiles) return;
+  const int64_t a0 = (int64_t)(blockIdx.x % d.tiles_a) * PRO_A, c0 = (int64_t)(blockIdx.x / d.tiles_a) * PRO_C;
+  nn_prologue_tile<BF>(d.P, rslv(ba, d.x), d.Dl, d.Dr, d.alpha, d.beta, rslv(ba, d.y), rslv(ba, d.xa_hi),
+                       BF ? rslv(ba, d.xa_lo) : nullptr, d.ka, rslv(ba, d.xb_hi), BF ? rslv(ba, d.xb_lo) : nullptr,
+                       d.kb, a0, c0, t);
 }
 
 BaseArgs to_args(const void* const base[NBASE]) {
@@ -299,5 +304,21 @@ mk_status phys2_group(const PhysItem* items, int n, int max_blocks, int d12, con
 }
 
 int phys2_blocks(int64_t Dl, int64_t Dr) { return (int)cdiv(Dl * Dr, (int64_t)TPB); }
+
+void nn_tiles(int64_t Dl, int64_t Dr, int32_t& tiles_a, int32_t& tiles) {
+  tiles_a = (int32_t)cdiv(Dl, (int64_t)PRO_A);
+  tiles = tiles_a * (int32_t)cdiv(Dr, (int64_t)PRO_C);
+}
+
+mk_status nn_group(const NNItem* items, int n, int max_tiles, bool bf16, const void* const base[NBASE],
+                   hipStream_t st) {
+  if (n == 0 || max_tiles == 0) return MK_OK;
+  MK_REQUIRE(n < 65536, MK_BAD_ARG, "nn_group: too many items");
+  const BaseArgs ba = to_args(base);
+  if (bf16) hipLaunchKernelGGL(nn_group_kernel<true>, dim3(max_tiles, n), dim3(256), 0, st, items, ba);
+  else hipLaunchKernelGGL(nn_group_kernel<false>, dim3(max_tiles, n), dim3(256), 0, st, items, ba);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
 
 }  // namespace mk

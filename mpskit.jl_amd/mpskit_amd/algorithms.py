@@ -133,28 +133,71 @@ def truncated_split(M, direction, trunc_dim, trunc_err, method="auto"):
         G = M @ M.t() if direction == "right" else M.t() @ M
         lam, Q = torch.linalg.eigh(0.5 * (G + G.t()))
         lam, Q = torch.flip(lam, [0]).clamp_min(0.0), torch.flip(Q, [1])
-        S = torch.sqrt(lam)
+        return _split_from_gram(M, direction, lam, Q, trunc_dim, trunc_err)
     k = min(trunc_dim, int(lam.shape[0]))
     if trunc_err > 0.0:
         tail = torch.sqrt(torch.flip(torch.cumsum(torch.flip(lam, [0]), 0), [0]))
         keep = int((tail > trunc_err).sum().item())
         k = max(1, min(k, keep))
     eps = float(torch.sqrt(lam[k:].sum())) if k < lam.shape[0] else 0.0
-    if method == "svd":
-        U, S, Vh = U[:, :k], S[:k], Vh[:k]
-        return (U, S[:, None] * Vh, S, eps) if direction == "right" else (U * S[None, :], Vh, S, eps)
+    U, S, Vh = U[:, :k], S[:k], Vh[:k]
+    return (U, S[:, None] * Vh, S, eps) if direction == "right" else (U * S[None, :], Vh, S, eps)
+
+
+def _split_from_gram(M, direction, lam, Q, trunc_dim, trunc_err):
+    """Rank-k split of M from the eigen-decomposition (lam descending, >= 0; Q the
+    eigenvectors) of its Gram matrix on the isometry's side: the isometry is
+    Q[:, :k] and the center the exact product with it."""
+    S = torch.sqrt(lam)
+    k = min(trunc_dim, int(lam.shape[0]))
+    if trunc_err > 0.0:
+        tail = torch.sqrt(torch.flip(torch.cumsum(torch.flip(lam, [0]), 0), [0]))
+        keep = int((tail > trunc_err).sum().item())
+        k = max(1, min(k, keep))
+    eps = float(torch.sqrt(lam[k:].sum())) if k < lam.shape[0] else 0.0
     Q = Q[:, :k].contiguous()
     if direction == "right":
         return Q, Q.t() @ M, S[:k], eps
     return M @ Q, Q.t().contiguous(), S[:k], eps
 
 
-def gauge2(psi: FiniteMPS, pos, direction, ac2, trunc_dim, trunc_err, method="auto"):
+def truncated_split_batched(Ms, direction, trunc_dim, trunc_err):
+    """``truncated_split(..., method="eigh")`` of many matrices at once.  The Gram
+    matrices are grouped by size (each group padded to its largest member) and
+    decomposed by one batched float64 eigh per group; padding carries -1 on the
+    diagonal, so its eigenvalues sort below every eigenvalue of the (positive
+    semidefinite) Gram matrix and its eigenvectors never mix with the kept ones."""
+    sides = [M if direction == "right" else M.t() for M in Ms]
+    order = sorted(range(len(Ms)), key=lambda i: -sides[i].shape[0])
+    out = [None] * len(Ms)
+    while order:
+        r0 = sides[order[0]].shape[0]
+        group = [i for i in order if sides[i].shape[0] >= 0.7 * r0]
+        order = [i for i in order if sides[i].shape[0] < 0.7 * r0]
+        dev = sides[group[0]].device
+        G = torch.zeros((len(group), r0, r0), dtype=torch.float64, device=dev)
+        for b, i in enumerate(group):
+            r = sides[i].shape[0]
+            g = sides[i] @ sides[i].t()
+            G[b, :r, :r] = 0.5 * (g + g.t())
+            if r < r0:
+                G[b, r:, r:].diagonal().fill_(-1.0)
+        lam, Q = torch.linalg.eigh(G)
+        lam, Q = torch.flip(lam, [1]), torch.flip(Q, [2])
+        for b, i in enumerate(group):
+            r = sides[i].shape[0]
+            out[i] = _split_from_gram(Ms[i], direction, lam[b, :r].clamp_min(0.0), Q[b, :r, :r], trunc_dim,
+                                      trunc_err)
+    return out
+
+
+def gauge2(psi: FiniteMPS, pos, direction, ac2, trunc_dim, trunc_err, method="auto", split=None):
     """Truncated split of the two-site center (``gauge2!``, orthoview.jl:401-420)
-    with ``normalize = true``; returns the discarded-weight 2-norm."""
+    with ``normalize = true``; returns the discarded-weight 2-norm.  ``split``:
+    a precomputed ``truncated_split`` result of the fused center."""
     Dl, d1, Dr, d2 = ac2.shape
     M = T.fuse(ac2, 2)
-    A, B, S, eps = truncated_split(M, direction, trunc_dim, trunc_err, method)
+    A, B, S, eps = split if split is not None else truncated_split(M, direction, trunc_dim, trunc_err, method)
     k = A.shape[1]
     # rows of B run over (r, p2) with r fastest -> tensor[s, p2, r]
     if direction == "right":
@@ -300,8 +343,13 @@ def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None
                                          precision=alg.precision, use_graph=use_graph)
             t_solve += time.perf_counter() - ts
             numops += max(i["numops"] for i in infos)
-            for psi, v in zip(psis, vecs):
-                gauge2(psi, pos, direction, v, alg.trunc_dim, alg.trunc_err, alg.gauge)
+            if alg.gauge == "svd":
+                splits = [None] * n
+            else:
+                splits = truncated_split_batched([T.fuse(v, 2) for v in vecs], direction, alg.trunc_dim,
+                                                 alg.trunc_err)
+            for psi, v, sp in zip(psis, vecs, splits):
+                gauge2(psi, pos, direction, v, alg.trunc_dim, alg.trunc_err, alg.gauge, split=sp)
             Bt.free()
             for op in ops:
                 op.free()

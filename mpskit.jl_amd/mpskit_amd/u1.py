@@ -108,14 +108,20 @@ def _middle_charges(left: U1Bond, right: U1Bond, phys_q):
 
 def _batched_gram_eigh(blocks):
     """Eigen-decompositions of the Gram matrices of a list of matrices (rows side),
-    one batched float64 eigh over zero-padded copies; eigenvalues descending."""
+    one batched float64 eigh over padded copies; eigenvalues descending.  The
+    padding carries -1 on its diagonal, so its eigenpairs sort after every
+    eigenvalue of the (positive semidefinite) Gram matrices and the first r
+    eigenvectors of a block of r rows have no component in its padding."""
     n = max(b.shape[0] for b in blocks)
     dev = blocks[0].device
     G = torch.zeros((len(blocks), n, n), dtype=torch.float64, device=dev)
     for k, b in enumerate(blocks):
         r = b.shape[0]
-        G[k, :r, :r] = b @ b.t()
-    lam, Q = torch.linalg.eigh(0.5 * (G + G.transpose(1, 2)))
+        g = b @ b.t()
+        G[k, :r, :r] = 0.5 * (g + g.t())
+        if r < n:
+            G[k, r:, r:].diagonal().fill_(-1.0)
+    lam, Q = torch.linalg.eigh(G)
     return torch.flip(lam, [1]).clamp_min(0.0), torch.flip(Q, [2])
 
 

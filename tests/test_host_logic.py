@@ -39,3 +39,34 @@ def test_finite_chain_boundaries():
     for g in range(8):
         for k in range(g):
             assert not np.any(Wj[g, :, :, k]), (g, k)
+
+
+def test_truncated_split_batched_matches_single_and_svd():
+    """Batched Gram-eigh splits (size groups, -1 padding) against the single
+    split and against the SVD: same kept singular values and discarded weight,
+    orthonormal isometries, and A B equal to the SVD's rank-k truncation."""
+    import torch
+    from mpskit_amd.algorithms import truncated_split, truncated_split_batched
+    g = torch.Generator().manual_seed(7)
+    shapes = [(40, 36), (64, 64), (24, 50), (61, 30), (64, 20)]
+    Ms = []
+    for m, n in shapes:
+        U, _ = torch.linalg.qr(torch.randn(m, m, generator=g, dtype=torch.float64))
+        V, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=torch.float64))
+        r = min(m, n)
+        s = torch.logspace(0, -6, r, dtype=torch.float64)   # distinct, decaying singular values
+        Ms.append(U[:, :r] @ torch.diag(s) @ V[:, :r].t())
+    for direction in ("right", "left"):
+        for trunc_dim, trunc_err in ((16, 0.0), (64, 1e-4)):
+            got = truncated_split_batched(Ms, direction, trunc_dim, trunc_err)
+            for M, (A, B, S, eps) in zip(Ms, got):
+                A1, B1, S1, e1 = truncated_split(M, direction, trunc_dim, trunc_err, method="svd")
+                assert S.shape == S1.shape
+                assert torch.allclose(S, S1, rtol=1e-6, atol=1e-12)
+                # discarded weight from Gram eigenvalues: absolute error ~ eps_mach * s_max^2 each
+                assert abs(eps ** 2 - e1 ** 2) <= 600 * 2.3e-16 * float(S1[0]) ** 2
+                iso = A if direction == "right" else B.t()
+                assert torch.allclose(iso.t() @ iso, torch.eye(iso.shape[1], dtype=torch.float64), atol=1e-12)
+                assert torch.allclose(A @ B, A1 @ B1, atol=1e-10)
+                S2 = truncated_split(M, direction, trunc_dim, trunc_err, method="eigh")[2]
+                assert torch.allclose(S, S2, atol=1e-12)
