@@ -303,6 +303,53 @@ def u1_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def sweep_workload(args):
+    """Where one DMRG2 sweep at D=2048 spends its time: the sweep of the default
+    line, run once per precision with host timers (device synchronised) around
+    the local eigensolves, the truncated splits, the QR of the local error
+    estimate, the environment updates and the operator preparation."""
+    import collections
+    import torch
+    import mpskit_amd as mk
+    alg_mod = sys.modules["mpskit_amd.algorithms"]
+    env_mod = sys.modules["mpskit_amd.environments"]
+    der = sys.modules["mpskit_amd.derivatives"]
+    acc, cnt = collections.defaultdict(float), collections.defaultdict(int)
+
+    def timed(obj, name, key):
+        f = getattr(obj, name)
+
+        def g(*a, **k):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = f(*a, **k)
+            torch.cuda.synchronize()
+            acc[key] += time.perf_counter() - t
+            cnt[key] += 1
+            return r
+        setattr(obj, name, g)
+
+    timed(der.PreparedDerivative, "eigsolve", "local eigensolves")
+    timed(alg_mod, "truncated_split", "truncated splits")
+    timed(alg_mod, "qr_left", "QR for the local error estimate")
+    timed(env_mod, "transfer_left", "environment updates")
+    timed(env_mod, "transfer_right", "environment updates")
+    timed(alg_mod, "prepare_operator", "operator preparation")
+    out = {"metric": "DMRG2 sweep time breakdown, Heisenberg chain L=24, bonds up to 2048", "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": "one DMRG2 sweep from a random normalised state (seed 0), Lanczos tol 1e-10"}}
+    for prec in ("fp64", "bf16x3"):
+        acc.clear()
+        cnt.clear()
+        sec, e, nops = dmrg_sweep(mk, prec)
+        out[prec] = {"seconds": sec, "energy_per_site": e, "applies": nops,
+                     "parts": {k: {"seconds": acc[k], "calls": cnt[k]} for k in sorted(acc, key=lambda k: -acc[k])}}
+        print(f"# {prec}: sweep {sec:.2f} s", file=sys.stderr, flush=True)
+    out["value"] = out["bf16x3"]["seconds"]
+    out["dtype"] = "bf16x3"
+    print(json.dumps(out), flush=True)
+
+
 def vumps_workload(args):
     """VUMPS on the infinite J1-J2 chain (J2/J1 = 0.24, unit cell 2, w = 8) at bond
     dimension D: seconds per iteration in float64 and in the mixed path (FP64
@@ -345,7 +392,7 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
                     help="time one precision only (for profiler runs); prints no JSON line")
     ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048")
-    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1"], default="apply")
+    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep"], default="apply")
     ap.add_argument("--vumps-D", type=int, default=1024)
     ap.add_argument("--u1-sweeps", type=int, default=2)
     ap.add_argument("--vumps-iters", type=int, default=6)
@@ -379,6 +426,9 @@ def main():
         return
     if args.workload == "u1":
         u1_workload(args)
+        return
+    if args.workload == "sweep":
+        sweep_workload(args)
         return
 
     D, d = args.D, 2
