@@ -350,6 +350,39 @@ def sweep_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def idmrg_workload(args):
+    """Single-site IDMRG on the J1-J2 chain of the VUMPS workload (J2/J1 = 0.24,
+    unit cell 2, D = --vumps-D), --vumps-iters iterations from one random start,
+    float64 and BF16x3 applies: seconds per iteration (iterations 2 on) and the
+    energy per site of the regauged result."""
+    import torch
+    import mpskit_amd as mk
+    from mpskit_amd import idmrg as I, vumps as V
+    D, iters = args.vumps_D, args.vumps_iters
+    H = V.InfiniteHamiltonian([mk.models.j1j2_bulk(1.0, 0.24)] * 2)
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        psi = V.InfiniteMPS.random(2, 2, D, seed=7)
+        torch.cuda.synchronize()
+        print(f"# idmrg {prec} D={D}", file=sys.stderr, flush=True)
+        psi, envs, eps, hist = I.find_groundstate_idmrg(psi, H, I.IDMRG(tol=0.0, maxiter=iters, precision=prec,
+                                                                        verbosity=2))
+        ts = [h["time"] for h in hist[1:]] or [hist[0]["time"]]
+        res[prec] = dict(seconds_per_iteration=sum(ts) / len(ts), energy_per_site=hist[-1]["energy"],
+                         boundary_change=eps, applies_per_iteration=sum(h["numops"] for h in hist) / len(hist),
+                         iterations=len(hist))
+    out = {"metric": "IDMRG time per iteration, infinite J1-J2 chain (J2/J1=0.24, cell 2, w=8)",
+           "value": res["bf16x3"]["seconds_per_iteration"], "unit": "s/iteration", "n_gpus": 1,
+           "steps": iters, "warmup": 1, "ms_per_step": res["bf16x3"]["seconds_per_iteration"] * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+           "data": "synthetic",
+           "config": {"workload": f"single-site IDMRG J1-J2 J2/J1=0.24 unit cell 2 D={D}, random start seed 7, "
+                                  f"{iters} iterations"},
+           "fp64": res["fp64"], "bf16x3": res["bf16x3"],
+           "speedup": res["fp64"]["seconds_per_iteration"] / res["bf16x3"]["seconds_per_iteration"]}
+    print(json.dumps(out), flush=True)
+
+
 def vumps_workload(args):
     """VUMPS on the infinite J1-J2 chain (J2/J1 = 0.24, unit cell 2, w = 8) at bond
     dimension D: seconds per iteration in float64 and in the mixed path (FP64
@@ -392,7 +425,7 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
                     help="time one precision only (for profiler runs); prints no JSON line")
     ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048")
-    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep"], default="apply")
+    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg"], default="apply")
     ap.add_argument("--vumps-D", type=int, default=1024)
     ap.add_argument("--u1-sweeps", type=int, default=2)
     ap.add_argument("--vumps-iters", type=int, default=6)
@@ -429,6 +462,9 @@ def main():
         return
     if args.workload == "sweep":
         sweep_workload(args)
+        return
+    if args.workload == "idmrg":
+        idmrg_workload(args)
         return
 
     D, d = args.D, 2
