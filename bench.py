@@ -383,6 +383,39 @@ def idmrg_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def idmrg2_workload(args):
+    """Two-site IDMRG on the J1-J2 chain of the VUMPS workload (J2/J1 = 0.24,
+    unit cell 2) growing from D = 16 to --idmrg2-D kept states, --vumps-iters
+    iterations, float64 and BF16x3 applies: seconds per iteration (iterations 2
+    on), bond dimension reached and the energy per site of the regauged result."""
+    import torch
+    import mpskit_amd as mk
+    from mpskit_amd import idmrg as I, vumps as V
+    D, iters = args.idmrg2_D, args.vumps_iters
+    H = V.InfiniteHamiltonian([mk.models.j1j2_bulk(1.0, 0.24)] * 2)
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        psi = V.InfiniteMPS.random(2, 2, 16, seed=7)
+        torch.cuda.synchronize()
+        print(f"# idmrg2 {prec} D={D}", file=sys.stderr, flush=True)
+        psi, envs, eps, hist = I.find_groundstate_idmrg2(
+            psi, H, I.IDMRG2(trunc_dim=D, tol=0.0, maxiter=iters, precision=prec, verbosity=2))
+        ts = [h["time"] for h in hist[1:]] or [hist[0]["time"]]
+        res[prec] = dict(seconds_per_iteration=sum(ts) / len(ts), energy_per_site=hist[-1]["energy"],
+                         boundary_change=eps, bond_dims=[h["D"] for h in hist],
+                         applies_per_iteration=sum(h["numops"] for h in hist) / len(hist), iterations=len(hist))
+    out = {"metric": "IDMRG2 time per iteration, infinite J1-J2 chain (J2/J1=0.24, cell 2, w=8)",
+           "value": res["bf16x3"]["seconds_per_iteration"], "unit": "s/iteration", "n_gpus": 1,
+           "steps": iters, "warmup": 1, "ms_per_step": res["bf16x3"]["seconds_per_iteration"] * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+           "data": "synthetic",
+           "config": {"workload": f"two-site IDMRG J1-J2 J2/J1=0.24 unit cell 2, D 16 -> {D}, random start seed 7, "
+                                  f"{iters} iterations"},
+           "fp64": res["fp64"], "bf16x3": res["bf16x3"],
+           "speedup": res["fp64"]["seconds_per_iteration"] / res["bf16x3"]["seconds_per_iteration"]}
+    print(json.dumps(out), flush=True)
+
+
 def vumps_workload(args):
     """VUMPS on the infinite J1-J2 chain (J2/J1 = 0.24, unit cell 2, w = 8) at bond
     dimension D: seconds per iteration in float64 and in the mixed path (FP64
@@ -425,10 +458,11 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
                     help="time one precision only (for profiler runs); prints no JSON line")
     ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048")
-    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg"], default="apply")
+    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg", "idmrg2"], default="apply")
     ap.add_argument("--vumps-D", type=int, default=1024)
     ap.add_argument("--u1-sweeps", type=int, default=2)
     ap.add_argument("--vumps-iters", type=int, default=6)
+    ap.add_argument("--idmrg2-D", type=int, default=256, help="kept states of the IDMRG2 workload")
     ap.add_argument("--L", type=int, default=48, help="chain length of the batched workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device-to-device five times (HBM counter calibration) and exit")
@@ -465,6 +499,9 @@ def main():
         return
     if args.workload == "idmrg":
         idmrg_workload(args)
+        return
+    if args.workload == "idmrg2":
+        idmrg2_workload(args)
         return
 
     D, d = args.D, 2
