@@ -16,6 +16,7 @@
 // in low precision, so every cycle contracts the float64 residual by about the
 // low-precision error and the loop stops on the float64 residual.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "kernels.h"
@@ -116,7 +117,15 @@ void lanczos_lowest(const std::vector<double>& G, int ld, int k, double& lam, st
 // inner Krylov stopping factor for the mixed modes: stop the low-precision
 // cycle once its estimated residual is below eta * (float64 residual at the
 // start of the cycle), i.e. aim for a few digits of contraction per cycle.
-static double inner_eta(int precision) { return precision == MK_FP64 ? 0.0 : 1e-3; }
+// MK_INNER_ETA overrides the factor (read once).
+static double inner_eta(int precision) {
+  static const double eta = [] {
+    const char* e = getenv("MK_INNER_ETA");
+    const double v = e ? atof(e) : 0.0;
+    return v > 0.0 && v < 1.0 ? v : 1e-3;
+  }();
+  return precision == MK_FP64 ? 0.0 : eta;
+}
 
 }  // namespace mk
 
