@@ -161,34 +161,59 @@ def _split_from_gram(M, direction, lam, Q, trunc_dim, trunc_err):
     return M @ Q, Q.t().contiguous(), S[:k], eps
 
 
-def truncated_split_batched(Ms, direction, trunc_dim, trunc_err):
-    """``truncated_split(..., method="eigh")`` of many matrices at once.  The Gram
-    matrices are grouped by size (each group padded to its largest member) and
-    decomposed by one batched float64 eigh per group; padding carries -1 on the
-    diagonal, so its eigenvalues sort below every eigenvalue of the (positive
-    semidefinite) Gram matrix and its eigenvectors never mix with the kept ones."""
-    sides = [M if direction == "right" else M.t() for M in Ms]
-    order = sorted(range(len(Ms)), key=lambda i: -sides[i].shape[0])
-    out = [None] * len(Ms)
-    while order:
-        r0 = sides[order[0]].shape[0]
-        group = [i for i in order if sides[i].shape[0] >= 0.7 * r0]
-        order = [i for i in order if sides[i].shape[0] < 0.7 * r0]
-        dev = sides[group[0]].device
-        G = torch.zeros((len(group), r0, r0), dtype=torch.float64, device=dev)
+def _eigh_batch_ms(n, b):
+    """Model of one batched float64 eigh of b matrices of order n (ms), fitted to
+    rocSOLVER on one MI355X: a per-call cost growing with n (launch-bound
+    tridiagonalisation) plus a per-matrix cost."""
+    return 2.0 + 0.02 * n + b * 10.5 * (n / 2048.0) ** 2.5
+
+
+def gram_eigh_grouped(mats):
+    """Eigen-decompositions of the Gram matrices M M^T of many matrices: a list
+    of (lam, Q) per matrix, eigenvalues descending (clamped at 0) and their
+    eigenvectors.  The matrices, sorted by row count, are cut into groups by a
+    dynamic programme over the batched-eigh cost model (per-call cost against
+    padding waste); every group is one batched float64 eigh over copies padded to
+    its largest size.  The padding carries -1 on its diagonal, so its eigenvalues
+    sort below every eigenvalue of the (positive semidefinite) Gram matrices and
+    the first r eigenvectors of an r-row matrix have no component in its padding."""
+    order = sorted(range(len(mats)), key=lambda i: -mats[i].shape[0])
+    sz = [mats[i].shape[0] for i in order]
+    m = len(order)
+    best, cut = [0.0] + [float("inf")] * m, [0] * (m + 1)
+    for e in range(1, m + 1):
+        for b in range(e):
+            c = best[b] + _eigh_batch_ms(sz[b], e - b)
+            if c < best[e]:
+                best[e], cut[e] = c, b
+    groups, e = [], m
+    while e > 0:
+        groups.append(order[cut[e]:e])
+        e = cut[e]
+    out = [None] * len(mats)
+    for group in groups:
+        r0 = mats[group[0]].shape[0]
+        G = torch.zeros((len(group), r0, r0), dtype=torch.float64, device=mats[group[0]].device)
         for b, i in enumerate(group):
-            r = sides[i].shape[0]
-            g = sides[i] @ sides[i].t()
+            r = mats[i].shape[0]
+            g = mats[i] @ mats[i].t()
             G[b, :r, :r] = 0.5 * (g + g.t())
             if r < r0:
                 G[b, r:, r:].diagonal().fill_(-1.0)
         lam, Q = torch.linalg.eigh(G)
         lam, Q = torch.flip(lam, [1]), torch.flip(Q, [2])
         for b, i in enumerate(group):
-            r = sides[i].shape[0]
-            out[i] = _split_from_gram(Ms[i], direction, lam[b, :r].clamp_min(0.0), Q[b, :r, :r], trunc_dim,
-                                      trunc_err)
+            r = mats[i].shape[0]
+            out[i] = (lam[b, :r].clamp_min(0.0), Q[b, :r, :r])
     return out
+
+
+def truncated_split_batched(Ms, direction, trunc_dim, trunc_err):
+    """``truncated_split(..., method="eigh")`` of many matrices at once, the Gram
+    matrices decomposed by :func:`gram_eigh_grouped`."""
+    sides = [M if direction == "right" else M.t() for M in Ms]
+    return [_split_from_gram(M, direction, lam, Q, trunc_dim, trunc_err)
+            for M, (lam, Q) in zip(Ms, gram_eigh_grouped(sides))]
 
 
 def gauge2(psi: FiniteMPS, pos, direction, ac2, trunc_dim, trunc_err, method="auto", split=None):

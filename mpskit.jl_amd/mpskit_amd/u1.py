@@ -107,22 +107,10 @@ def _middle_charges(left: U1Bond, right: U1Bond, phys_q):
 
 
 def _batched_gram_eigh(blocks):
-    """Eigen-decompositions of the Gram matrices of a list of matrices (rows side),
-    one batched float64 eigh over padded copies; eigenvalues descending.  The
-    padding carries -1 on its diagonal, so its eigenpairs sort after every
-    eigenvalue of the (positive semidefinite) Gram matrices and the first r
-    eigenvectors of a block of r rows have no component in its padding."""
-    n = max(b.shape[0] for b in blocks)
-    dev = blocks[0].device
-    G = torch.zeros((len(blocks), n, n), dtype=torch.float64, device=dev)
-    for k, b in enumerate(blocks):
-        r = b.shape[0]
-        g = b @ b.t()
-        G[k, :r, :r] = 0.5 * (g + g.t())
-        if r < n:
-            G[k, r:, r:].diagonal().fill_(-1.0)
-    lam, Q = torch.linalg.eigh(G)
-    return torch.flip(lam, [1]).clamp_min(0.0), torch.flip(Q, [2])
+    """Eigen-decompositions of the Gram matrices of a list of matrices (rows side):
+    per block (eigenvalues descending, eigenvectors), batched over size groups."""
+    from .algorithms import gram_eigh_grouped
+    return gram_eigh_grouped(blocks)
 
 
 class U1FiniteMPS:
@@ -224,9 +212,11 @@ def u1_split(M, left: U1Bond, right: U1Bond, direction, trunc_dim, phys_q=PHYS_Q
     blocks = [M[r[:, None], cl[None, :]] for r, cl in zip(rows, cols)]
     side = [b if direction == "right" else b.t() for b in blocks]
     keep = [i for i, b in enumerate(side) if b.shape[0] > 0 and b.shape[1] > 0]
-    lam, Q = _batched_gram_eigh([side[i] for i in keep])
+    eig = _batched_gram_eigh([side[i] for i in keep])
+    lam = [e[0] for e in eig]
+    Q = [e[1] for e in eig]
     # global truncation over all blocks
-    allv = torch.cat([lam[k, :side[i].shape[0]] for k, i in enumerate(keep)])
+    allv = torch.cat([lam[k][:side[i].shape[0]] for k, i in enumerate(keep)])
     order = torch.argsort(allv, descending=True)
     kmax = min(trunc_dim, int(allv.numel()))
     if trunc_err > 0.0:
@@ -238,13 +228,13 @@ def u1_split(M, left: U1Bond, right: U1Bond, direction, trunc_dim, phys_q=PHYS_Q
     counts = []
     taken = 0
     for k, i in enumerate(keep):
-        v = lam[k, :side[i].shape[0]]
+        v = lam[k][:side[i].shape[0]]
         n = int((v >= thr).sum().item()) if thr > 0 else int((v > 0).sum().item())
         counts.append(n)
         taken += n
     # ties at the threshold may overshoot: trim from the smallest-eigenvalue sectors
     while taken > kmax:
-        k = min((k for k in range(len(keep)) if counts[k] > 0), key=lambda k: float(lam[k, counts[k] - 1]))
+        k = min((k for k in range(len(keep)) if counts[k] > 0), key=lambda k: float(lam[k][counts[k] - 1]))
         counts[k] -= 1
         taken -= 1
     new_q, new_d, parts = [], [], []
@@ -264,7 +254,7 @@ def u1_split(M, left: U1Bond, right: U1Bond, direction, trunc_dim, phys_q=PHYS_Q
         B = torch.zeros((Dn, n_), dtype=torch.float64, device=dev)
         for s, (i, k, n) in enumerate(parts):
             r = rows[i]
-            U = Q[k, :r.numel(), :n]
+            U = Q[k][:r.numel(), :n]
             A[r[:, None], torch.arange(offs[s], offs[s] + n, device=dev)[None, :]] = U
             B[offs[s]:offs[s] + n][:, cols[i]] = U.t() @ blocks[i]
         return A, B, bond, eps
@@ -272,7 +262,7 @@ def u1_split(M, left: U1Bond, right: U1Bond, direction, trunc_dim, phys_q=PHYS_Q
     B = torch.zeros((Dn, n_), dtype=torch.float64, device=dev)
     for s, (i, k, n) in enumerate(parts):
         cl = cols[i]
-        V = Q[k, :cl.numel(), :n]
+        V = Q[k][:cl.numel(), :n]
         B[torch.arange(offs[s], offs[s] + n, device=dev)[:, None], cl[None, :]] = V.t()
         A[rows[i][:, None], torch.arange(offs[s], offs[s] + n, device=dev)[None, :]] = blocks[i] @ V
     return A, B, bond, eps
