@@ -141,15 +141,32 @@ void gemm_f64_kernel(Src src) {
   for (int s = 0; s < P.nseg; ++s) nsteps += (P.seg[s].K + BK - 1) / BK;
 
   dx2 ra[CHA], rb[CHB];
+  // descriptor of the segment being loaded, read from the problem array once
+  // per segment rather than once per K step
+  struct Cur {
+    const double* A;
+    const double* B;
+    int64_t lda, ldb;
+    int K;
+    bool vec;
+  } cur{};
+  int cur_s = -1;
+  const int M = P.M, N = P.N;
   auto load = [&](int s, int k0) {
-    const GemmSeg& S = P.seg[s];
-    const bool vec = S.vec != 0;
-    const double* A = static_cast<const double*>(resolve(src, S.A));
-    const double* B = static_cast<const double*>(resolve(src, S.B));
-    if (!TA) load_fastdir<BM, LDA_S, BK>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
-    else     load_kdir<BM, LDA_S, BK>(A, S.lda, m0, k0, P.M, S.K, vec, ra);
-    if (!TB) load_kdir<BN, LDB_S, BK>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
-    else     load_fastdir<BN, LDB_S, BK>(B, S.ldb, n0, k0, P.N, S.K, vec, rb);
+    if (s != cur_s) {
+      const GemmSeg& S = P.seg[s];
+      cur.A = static_cast<const double*>(resolve(src, S.A));
+      cur.B = static_cast<const double*>(resolve(src, S.B));
+      cur.lda = S.lda;
+      cur.ldb = S.ldb;
+      cur.K = S.K;
+      cur.vec = S.vec != 0;
+      cur_s = s;
+    }
+    if (!TA) load_fastdir<BM, LDA_S, BK>(cur.A, cur.lda, m0, k0, M, cur.K, cur.vec, ra);
+    else     load_kdir<BM, LDA_S, BK>(cur.A, cur.lda, m0, k0, M, cur.K, cur.vec, ra);
+    if (!TB) load_kdir<BN, LDB_S, BK>(cur.B, cur.ldb, n0, k0, N, cur.K, cur.vec, rb);
+    else     load_fastdir<BN, LDB_S, BK>(cur.B, cur.ldb, n0, k0, N, cur.K, cur.vec, rb);
   };
   auto store = [&](int buf) {
     if (!TA) store_fastdir<BM, LDA_S, BK>(As[buf], ra);
@@ -165,9 +182,14 @@ void gemm_f64_kernel(Src src) {
   }
   __syncthreads();
   int buf = 0;
+  int kcur = nsteps > 0 ? P.seg[0].K : 0;
   for (int step = 0; step < nsteps; ++step) {
     int ns = s, nk0 = k0 + BK;
-    if (nk0 >= P.seg[s].K) { ns = s + 1; nk0 = 0; }
+    if (nk0 >= kcur) {
+      ns = s + 1;
+      nk0 = 0;
+      kcur = ns < P.nseg ? P.seg[ns].K : 0;
+    }
     const bool has_next = step + 1 < nsteps;
     if (has_next) load(ns, nk0);
 #pragma unroll
