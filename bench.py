@@ -110,6 +110,56 @@ def vendor_gemm_reference(D, d=2, reps=20):
     return out
 
 
+def batched_apply(mk, T, reps=10):
+    """The grouped (batched) dAC2 path: one mk_batch_apply applies every
+    instance of a batch in one grouped persistent GEMM launch plus one memory-
+    bound prologue.  Two batches of XXZ-like Jordan operators (random C/B
+    blocks, unit-norm random GL/GR, as in the isolated apply): 4 instances at
+    D = 2048 and 20 instances at the parameter-sweep sizes D_i = 256..1024.
+    Returns instance applies/s per precision and the BF16x3 / FP64 ratio."""
+    import torch
+    out = {}
+    cases = {"4 x D=2048": [2048] * 4,
+             "20 x D=256..1024": [256 + (768 * i // 19) // 16 * 16 for i in range(20)]}
+    for name, Ds in cases.items():
+        ops = []
+        for i, D in enumerate(Ds):
+            W = T.from_numpy(jordan_w(np.random.default_rng(100 + i)))
+            GL = gpu_unit(T, (D, 5, D), 200 + i)
+            GR = gpu_unit(T, (D, 5, D), 300 + i)
+            ops.append(mk.prepare_operator(mk.JordanMPO_AC2_Hamiltonian(GL, W, W, GR),
+                                           precisions=("fp64", "bf16x3")))
+            del GL, GR
+        B = mk.BatchedDerivative(ops)
+        xs = [gpu_unit(T, (D, 2, D, 2), 400 + i) for i, D in enumerate(Ds)]
+        flat = B.pack(xs)
+        res = {}
+        y64 = None
+        for prec in ("fp64", "bf16x3"):
+            y = B.apply(flat, prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                y = B.apply(flat, prec)
+            torch.cuda.synchronize()
+            sec = (time.perf_counter() - t0) / reps
+            if prec == "fp64":
+                y64 = y
+            err = float(torch.linalg.vector_norm(y - y64) / torch.linalg.vector_norm(y64))
+            res[prec] = dict(ms_per_batch=sec * 1e3, instance_applies_per_s=len(Ds) / sec, rel_err_vs_fp64=err)
+        res["speedup_vs_fp64"] = res["fp64"]["ms_per_batch"] / res["bf16x3"]["ms_per_batch"]
+        res["grouped"] = B.grouped
+        out[name] = res
+        print(f"# batched {name}: fp64 {res['fp64']['ms_per_batch']:.3f} ms, bf16x3 "
+              f"{res['bf16x3']['ms_per_batch']:.3f} ms per batch ({res['speedup_vs_fp64']:.2f}x)",
+              file=sys.stderr, flush=True)
+        B.free()
+        for op in ops:
+            op.free()
+        del xs, flat, y, y64
+    return out
+
+
 def cpu_baseline(D, budget_s=20.0):
     """float64 oracle restatement (numpy/OpenBLAS) of the same apply on host cores."""
     from threadpoolctl import threadpool_info, threadpool_limits
@@ -573,6 +623,8 @@ def main():
     }
     print("# vendor GEMM reference ...", file=sys.stderr, flush=True)
     out["vendor_gemm_reference"] = vendor_gemm_reference(D)
+    print("# batched applies ...", file=sys.stderr, flush=True)
+    out["batched_apply"] = batched_apply(mk, T)
     if not args.no_sweep:
         sw = {}
         for prec in ("fp64", "bf16x3"):

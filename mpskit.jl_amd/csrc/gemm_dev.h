@@ -2,7 +2,8 @@
 // `for (t = first(); t < total_tiles; t += stride())` and fetches the problem
 // owning tile t.  GemmBatch launches one workgroup per tile (the loop runs once,
 // tiles remapped so that each XCD gets a contiguous range); GemmGroup launches a
-// persistent grid over a device-resident problem array.
+// persistent grid over a device-resident problem array (remapped the same way
+// within every round of gridDim.x tiles).
 #pragma once
 #include "gemm.h"
 
@@ -27,7 +28,9 @@ __device__ __forceinline__ const GemmProb& src_prob(const GemmBatch& b, int t) {
   return b.p[pi];
 }
 
-__device__ __forceinline__ int src_first(const GemmGroup&) { return blockIdx.x; }
+// Persistent grid: round r covers tiles [r G, r G + G) (G = gridDim.x); the
+// remap gives each XCD a contiguous range of every round.
+__device__ __forceinline__ int src_first(const GemmGroup&) { return xcd_remap(blockIdx.x, gridDim.x); }
 __device__ __forceinline__ int src_stride(const GemmGroup&) { return gridDim.x; }
 __device__ __forceinline__ const GemmProb& src_prob(const GemmGroup& g, int t) {
   int lo = 0, hi = g.nprob - 1;
