@@ -38,10 +38,17 @@ __device__ __forceinline__ void nn_prologue_tile(const double* __restrict__ P, c
                                                  void* __restrict__ xa_lo, int64_t ka, void* __restrict__ xb_hi,
                                                  void* __restrict__ xb_lo, int64_t kb, int64_t a0, int64_t c0,
                                                  ProTile& t) {
+  // all arrays are in device memory: access them through global-typed pointers
+  typedef __attribute__((address_space(1))) double gdouble;
+  const gdouble* __restrict__ gx = (const gdouble*)x;
+  gdouble* __restrict__ gy = (gdouble*)y;
+  typedef __attribute__((address_space(1))) float gf;
+  typedef __attribute__((address_space(1))) unsigned short gus;
+  typedef __attribute__((address_space(1))) unsigned int gui;
   const int64_t M = 2 * Dl;
   double p[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) p[i] = P[i];
+  for (int i = 0; i < 16; ++i) p[i] = ((const gdouble*)P)[i];
   {
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // a offset, c phase (8)
     const int64_t a = a0 + tx;
@@ -52,13 +59,13 @@ __device__ __forceinline__ void nn_prologue_tile(const double* __restrict__ P, c
       if (a < Dl && c < Dr) {
         double xv[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) xv[q] = x[a + Dl * (q & 1) + M * (c + Dr * (q >> 1))];
+        for (int q = 0; q < 4; ++q) xv[q] = gx[a + Dl * (q & 1) + M * (c + Dr * (q >> 1))];
 #pragma unroll
         for (int o = 0; o < 4; ++o) {
           double s = 0.0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) s += p[(o & 1) + 2 * ((o >> 1) + 2 * q)] * xv[q];
-          double* yp = y + a + Dl * (o & 1) + M * (c + Dr * (o >> 1));
+          gdouble* yp = gy + a + Dl * (o & 1) + M * (c + Dr * (o >> 1));
           double v = alpha * s;
           if (beta != 0.0) v += beta * *yp;
           *yp = v;
@@ -70,13 +77,13 @@ __device__ __forceinline__ void nn_prologue_tile(const double* __restrict__ P, c
           if constexpr (SPLIT) {
             __hip_bfloat16 h, l;
             split2(xv[q], h, l);
-            static_cast<__hip_bfloat16*>(xb_hi)[cl * kb + rr] = h;
-            static_cast<__hip_bfloat16*>(xb_lo)[cl * kb + rr] = l;
+            ((gus*)xb_hi)[cl * kb + rr] = __builtin_bit_cast(unsigned short, h);
+            ((gus*)xb_lo)[cl * kb + rr] = __builtin_bit_cast(unsigned short, l);
             packed = (unsigned int)__builtin_bit_cast(unsigned short, h) |
                      ((unsigned int)__builtin_bit_cast(unsigned short, l) << 16);
           } else {
             const float f = (float)xv[q];
-            static_cast<float*>(xb_hi)[cl * kb + rr] = f;
+            ((gf*)xb_hi)[cl * kb + rr] = f;
             packed = __float_as_uint(f);
           }
           t[q][cc][tx] = packed;
@@ -98,10 +105,10 @@ __device__ __forceinline__ void nn_prologue_tile(const double* __restrict__ P, c
         const unsigned int v = t[q][tx][aa];
         const int64_t idx = (ar + Dl * (q & 1)) * ka + c + Dr * (q >> 1);
         if constexpr (SPLIT) {
-          static_cast<unsigned short*>(xa_hi)[idx] = (unsigned short)(v & 0xffffu);
-          static_cast<unsigned short*>(xa_lo)[idx] = (unsigned short)(v >> 16);
+          ((gus*)xa_hi)[idx] = (unsigned short)(v & 0xffffu);
+          ((gus*)xa_lo)[idx] = (unsigned short)(v >> 16);
         } else {
-          static_cast<unsigned int*>(xa_hi)[idx] = v;
+          ((gui*)xa_hi)[idx] = v;
         }
       }
     }

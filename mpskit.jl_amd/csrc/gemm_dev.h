@@ -45,12 +45,28 @@ __device__ __forceinline__ const GemmProb& src_prob(const GemmGroup& g, int t) {
 // Resolve a descriptor pointer: absolute for GemmBatch, (slot, offset) for GemmGroup.
 template <class T>
 __device__ __forceinline__ T* resolve(const GemmBatch&, T* p) { return p; }
+// Every operand lives in device (global) memory.  A pointer rebuilt from an
+// integer loses that fact and would be accessed with flat instructions, whose
+// completion counts on both the memory and the LDS counters; the round trip
+// through the global address space lets the compiler emit global loads and
+// stores again.
+template <class T>
+__device__ __forceinline__ T* as_global(T* p) {
+  return (T*)((__attribute__((address_space(1))) T*)p);
+}
+// The same pointer typed in the global address space, for the access sites.
+template <class T>
+using gptr_t = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr_t<T> gptr(T* p) {
+  return (gptr_t<T>)p;
+}
 template <class T>
 __device__ __forceinline__ T* resolve(const GemmGroup& g, T* p) {
   const uint64_t v = reinterpret_cast<uint64_t>(p);
   const int slot = (int)(v >> 62);
-  if (slot == 0) return p;
-  return reinterpret_cast<T*>(const_cast<char*>(g.base[slot]) + (v & REL_MASK));
+  T* r = slot == 0 ? p : reinterpret_cast<T*>(const_cast<char*>(g.base[slot]) + (v & REL_MASK));
+  return as_global(r);
 }
 
 // Position of tile u of a problem: tile rows are grouped GROUP_M at a time so

@@ -21,8 +21,8 @@ constexpr int PAD = 16;     // LDS row padding (doubles): alternates 128-B bank 
 
 // Stage a tile whose contiguous memory direction is the LDS fast direction.
 // element (r, k) lives at p[(r0 + r) + (k0 + k) * ld]; LDS tile S[k][r].
-template <int R, int LDR, int BK>
-__device__ __forceinline__ void load_fastdir(const double* __restrict__ p, int64_t ld, int r0,
+template <int R, int LDR, int BK, class Ptr>
+__device__ __forceinline__ void load_fastdir(Ptr __restrict__ p, int64_t ld, int r0,
                                              int k0, int Rlim, int Klim, bool vec,
                                              dx2 (&reg)[R * BK / 2 / NT]) {
   constexpr int CH = R * BK / 2 / NT;
@@ -34,9 +34,9 @@ __device__ __forceinline__ void load_fastdir(const double* __restrict__ p, int64
     const int gr = r0 + r, gk = k0 + k;
     dx2 v = {0.0, 0.0};
     if (gk < Klim) {
-      const double* q = p + (int64_t)gr + (int64_t)gk * ld;
+      const auto q = p + (int64_t)gr + (int64_t)gk * ld;
       if (vec && gr + 1 < Rlim) {
-        v = *reinterpret_cast<const dx2*>(q);
+        v = *(const __attribute__((address_space(1))) dx2*)q;
       } else {
         if (gr < Rlim) v.x = q[0];
         if (gr + 1 < Rlim) v.y = q[1];
@@ -66,8 +66,8 @@ __device__ __forceinline__ void store_fastdir(double (*S)[LDR], const dx2 (&reg)
 // aligned 32-column group, undone by the fragment reads).
 __device__ __forceinline__ int kswz(int k) { return k & 30; }
 
-template <int R, int LDR, int BK>
-__device__ __forceinline__ void load_kdir(const double* __restrict__ p, int64_t ld, int r0, int k0,
+template <int R, int LDR, int BK, class Ptr>
+__device__ __forceinline__ void load_kdir(Ptr __restrict__ p, int64_t ld, int r0, int k0,
                                           int Rlim, int Klim, bool vec,
                                           dx2 (&reg)[R * BK / 2 / NT]) {
   constexpr int CH = R * BK / 2 / NT;
@@ -79,9 +79,9 @@ __device__ __forceinline__ void load_kdir(const double* __restrict__ p, int64_t 
     const int gr = r0 + r, gk = k0 + k;
     dx2 v = {0.0, 0.0};
     if (gr < Rlim) {
-      const double* q = p + (int64_t)gk + (int64_t)gr * ld;
+      const auto q = p + (int64_t)gk + (int64_t)gr * ld;
       if (vec && gk + 1 < Klim) {
-        v = *reinterpret_cast<const dx2*>(q);
+        v = *(const __attribute__((address_space(1))) dx2*)q;
       } else {
         if (gk < Klim) v.x = q[0];
         if (gk + 1 < Klim) v.y = q[1];
@@ -143,9 +143,10 @@ void gemm_f64_kernel(Src src) {
   dx2 ra[CHA], rb[CHB];
   // descriptor of the segment being loaded, read from the problem array once
   // per segment rather than once per K step
+  typedef const __attribute__((address_space(1))) double* gptr;   // operands are in device memory
   struct Cur {
-    const double* A;
-    const double* B;
+    gptr A;
+    gptr B;
     int64_t lda, ldb;
     int K;
     bool vec;
@@ -155,8 +156,8 @@ void gemm_f64_kernel(Src src) {
   auto load = [&](int s, int k0) {
     if (s != cur_s) {
       const GemmSeg& S = P.seg[s];
-      cur.A = static_cast<const double*>(resolve(src, S.A));
-      cur.B = static_cast<const double*>(resolve(src, S.B));
+      cur.A = (gptr)static_cast<const double*>(resolve(src, S.A));
+      cur.B = (gptr)static_cast<const double*>(resolve(src, S.B));
       cur.lda = S.lda;
       cur.ldb = S.ldb;
       cur.K = S.K;
@@ -220,7 +221,7 @@ void gemm_f64_kernel(Src src) {
   // block is C^T, so the lane holds C[row li][col lk + 4 r]: the 16 lanes of a
   // quarter-wave store 16 consecutive rows (128 B) of one column.
   const double alpha = P.alpha, beta = P.beta;
-  double* Cp = resolve(src, P.C);
+  __attribute__((address_space(1))) double* Cp = (__attribute__((address_space(1))) double*)resolve(src, P.C);
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
     const int row = m0 + wm0 + i * 16 + li;
@@ -231,7 +232,7 @@ void gemm_f64_kernel(Src src) {
       for (int r = 0; r < 4; ++r) {
         const int col = n0 + wn0 + j * 16 + lk + 4 * r;
         if (col >= P.N) continue;
-        double* cp = Cp + (int64_t)row + (int64_t)col * P.ldc;
+        __attribute__((address_space(1))) double* cp = Cp + (int64_t)row + (int64_t)col * P.ldc;
         double v = alpha * acc[i][j][r];
         if (beta != 0.0) v += beta * *cp;
         *cp = v;

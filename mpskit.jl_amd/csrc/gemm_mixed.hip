@@ -65,7 +65,7 @@ __device__ __forceinline__ void store_tile(const fx4 (&acc)[FM][FN], double* __r
       for (int r = 0; r < 4; ++r) {
         const int col = col0 + j * 16 + lg * 4 + r;
         if (col >= N) continue;
-        double* cp = C + (int64_t)row + (int64_t)col * ldc;
+        gptr_t<double> cp = gptr(C) + (int64_t)row + (int64_t)col * ldc;
         double v = alpha * (double)acc[i][j][r];
         if (beta != 0.0) v += beta * *cp;
         *cp = v;
@@ -89,7 +89,7 @@ __device__ __forceinline__ void load_slab(const char* __restrict__ base, int64_t
     const int gr = r0 + r;
     const int64_t kb = kbyte0 + q * 16;
     ux4 v = {0u, 0u, 0u, 0u};
-    if (gr < Rlim && kb < Kbytes) v = *reinterpret_cast<const ux4*>(base + (int64_t)gr * ld_bytes + kb);
+    if (gr < Rlim && kb < Kbytes) v = *gptr(reinterpret_cast<const ux4*>(base + (int64_t)gr * ld_bytes + kb));
     reg[c] = v;
   }
 }
@@ -248,7 +248,7 @@ __device__ __forceinline__ void load_slab_big(const char* __restrict__ base, int
     const int gr = r0 + r;
     const int64_t kb = kbyte0 + q * 16;
     ux4 v = {0u, 0u, 0u, 0u};
-    if (gr < Rlim && kb < Kbytes) v = *reinterpret_cast<const ux4*>(base + (int64_t)gr * ld_bytes + kb);
+    if (gr < Rlim && kb < Kbytes) v = *gptr(reinterpret_cast<const ux4*>(base + (int64_t)gr * ld_bytes + kb));
     reg[c] = v;
   }
 }

@@ -23,12 +23,15 @@ struct BaseArgs {
   const char* b[NBASE];
 };
 
+// Operands live in device memory: the round trip through the global address
+// space keeps the compiler from accessing a pointer rebuilt from an integer
+// with flat instructions.
 template <class T>
 __device__ __forceinline__ T* rslv(const BaseArgs& ba, T* p) {
   const uint64_t v = reinterpret_cast<uint64_t>(p);
   const int slot = (int)(v >> 62);
-  if (slot == 0) return p;
-  return reinterpret_cast<T*>(const_cast<char*>(ba.b[slot]) + (v & REL_MASK));
+  T* r = slot == 0 ? p : reinterpret_cast<T*>(const_cast<char*>(ba.b[slot]) + (v & REL_MASK));
+  return (T*)((__attribute__((address_space(1))) T*)r);
 }
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -189,8 +192,10 @@ __global__ void phys2_group_kernel(const PhysItem* __restrict__ items, BaseArgs 
   const int64_t n = d.Dl * d.Dr;
   const int64_t idx = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (idx >= n) return;
-  const double* x = rslv(ba, d.x);
-  double* y = rslv(ba, d.y);
+  typedef __attribute__((address_space(1))) double gdouble;
+  const gdouble* x = (const gdouble*)rslv(ba, d.x);
+  gdouble* y = (gdouble*)rslv(ba, d.y);
+  const gdouble* P = (const gdouble*)d.P;
   const int d1 = d.d1, d2 = d.d2;
   const int64_t a = idx % d.Dl, c = idx / d.Dl;
   const int64_t base = a + d.Dl * d1 * c;
@@ -203,8 +208,8 @@ __global__ void phys2_group_kernel(const PhysItem* __restrict__ items, BaseArgs 
     const int b = o % d1, e = o / d1;
     double s = 0.0;
 #pragma unroll
-    for (int t = 0; t < D12; ++t) s += d.P[b + d1 * (e + d2 * t)] * xv[t];
-    double* yp = y + base + d.Dl * b + sx * e;
+    for (int t = 0; t < D12; ++t) s += P[b + d1 * (e + d2 * t)] * xv[t];
+    gdouble* yp = y + base + d.Dl * b + sx * e;
     double v = d.alpha * s;
     if (d.beta != 0.0) v += d.beta * *yp;
     *yp = v;
