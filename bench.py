@@ -26,6 +26,7 @@ Other workloads print their own JSON line:
     --workload u1        U(1) XXZ DMRG2, L = 64, D = 2048 over 21 charge sectors
                          (charge-block grouped GEMMs, captured Krylov steps)
     --workload vumps     VUMPS on the J1-J2 chain, cell 2, D = 1024
+    --workload boundary  leading_boundary (VUMPS) of the 2D Ising transfer MPO, D = 1024
 """
 from __future__ import annotations
 
@@ -498,6 +499,43 @@ def vumps_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def boundary_workload(args):
+    """leading_boundary of the square-lattice Ising transfer MPO (beta = 0.42,
+    disordered side of beta_c = 0.4407, unit cell 1, w = d = 2) with VUMPS at bond
+    dimension --boundary-D: --vumps-iters iterations from one random state in
+    float64 and in the mixed path (AC / C eigensolves in BF16x3 refined in FP64;
+    environments, gauge and the Galerkin error in FP64).  Reports seconds per
+    iteration (iterations 2 on) and ln Z per site against Onsager's value."""
+    import torch
+    import mpskit_amd as mk
+    from mpskit_amd import boundary as Bd, vumps as V
+    D, iters, beta = args.boundary_D, args.vumps_iters, 0.42
+    O = Bd.InfiniteMPO([mk.models.classical_ising(beta)])
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        psi = V.InfiniteMPS.random(1, 2, D, seed=7)
+        torch.cuda.synchronize()
+        print(f"# boundary {prec} D={D}", file=sys.stderr, flush=True)
+        _, _, eps, hist = Bd.leading_boundary(psi, O, Bd.BoundaryVUMPS(tol=1e-12, maxiter=iters, precision=prec,
+                                                                       verbosity=2))
+        per = [h["time"] for h in hist[1:]] or [hist[0]["time"]]
+        res[prec] = dict(seconds_per_iteration=sum(per) / len(per), log_z_per_site=float(np.log(hist[-1]["value"])),
+                         galerkin=eps, iterations=len(hist), mixed_applies=hist[-1]["mixed_ops"],
+                         fp64_applies=hist[-1]["fp64_ops"])
+    out = {
+        "metric": "leading_boundary VUMPS time per iteration, 2D Ising transfer MPO (beta=0.42, w=2)",
+        "value": res["bf16x3"]["seconds_per_iteration"], "unit": "s/iteration", "n_gpus": 1, "steps": iters,
+        "warmup": 1, "ms_per_step": res["bf16x3"]["seconds_per_iteration"] * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3", "data": "synthetic",
+        "config": {"workload": f"VUMPS leading boundary, classical Ising beta=0.42, unit cell 1, D={D}, "
+                               f"random start seed 7, {iters} iterations"},
+        "fp64": res["fp64"], "bf16x3": res["bf16x3"],
+        "speedup": res["fp64"]["seconds_per_iteration"] / res["bf16x3"]["seconds_per_iteration"],
+        "log_z_diff": abs(res["fp64"]["log_z_per_site"] - res["bf16x3"]["log_z_per_site"]),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -508,11 +546,12 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
                     help="time one precision only (for profiler runs); prints no JSON line")
     ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048")
-    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg", "idmrg2"], default="apply")
+    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg", "idmrg2", "boundary"], default="apply")
     ap.add_argument("--vumps-D", type=int, default=1024)
     ap.add_argument("--u1-sweeps", type=int, default=2)
     ap.add_argument("--vumps-iters", type=int, default=6)
     ap.add_argument("--idmrg2-D", type=int, default=256, help="kept states of the IDMRG2 workload")
+    ap.add_argument("--boundary-D", type=int, default=1024, help="bond dimension of the boundary workload")
     ap.add_argument("--L", type=int, default=48, help="chain length of the batched workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device-to-device five times (HBM counter calibration) and exit")
@@ -552,6 +591,9 @@ def main():
         return
     if args.workload == "idmrg2":
         idmrg2_workload(args)
+        return
+    if args.workload == "boundary":
+        boundary_workload(args)
         return
 
     D, d = args.D, 2

@@ -95,3 +95,17 @@ def xxz(L, delta=1.0):
 
 def j1j2(L, j1=1.0, j2=0.24):
     return finite_chain(j1j2_bulk(j1, j2), L)
+
+
+def classical_ising(beta=float(np.log(1.0 + np.sqrt(2.0)) / 2)):
+    """Row-to-row transfer MPO tensor of the square-lattice Ising model (J = 1),
+    ``O[g, b, h, k]`` (left, out, in, right): a delta tensor on the site with the
+    symmetric square root of the bond Boltzmann matrix [[e^b, e^-b], [e^-b, e^b]]
+    on each of its four legs (classical_ising_tensors / classical_ising,
+    test/setup/testsetup.jl:193-214, 232-234).  The per-site dominant eigenvalue
+    of the resulting InfiniteMPO is the partition function per site."""
+    t = np.array([[np.exp(beta), np.exp(-beta)], [np.exp(-beta), np.exp(beta)]])
+    ev, U = np.linalg.eigh(t)
+    nt = U @ np.diag(np.sqrt(ev)) @ U.T
+    return np.einsum("gs,bs,hs,ks->gbhk", nt, nt, nt, nt)
+

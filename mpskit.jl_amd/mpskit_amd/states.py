@@ -41,23 +41,68 @@ def _mat(A, rows):
     return T.fuse(A, rows)
 
 
+_EPS = float(np.finfo(np.float64).eps) / 2   # unit roundoff
+
+
+def _householder_pos_qr(M):
+    Q, R = torch.linalg.qr(M)
+    s = torch.sign(torch.diagonal(R))
+    s[s == 0] = 1
+    return Q * s, s[:, None] * R
+
+
+def _chol_step(Q, shift=0.0):
+    """One Cholesky QR pass: G = Q^T Q (+ shift I) = R^T R, Q <- Q R^-1."""
+    G = Q.t() @ Q
+    if shift:
+        G.diagonal().add_(shift)
+    L, info = torch.linalg.cholesky_ex(G)
+    return L, info
+
+
+def pos_qr(M):
+    """Thin QR of an m x n float64 GPU matrix (m >= n) with a positive diagonal
+    in R (the qr_compact(; positive = true) of the reference's gauge steps,
+    ortho.jl:307-340), by shifted Cholesky QR3 (Fukaya, Kannan, Nakatsukasa,
+    Zhang, Yamamoto, SIAM J. Sci. Comput. 42 (2020) A477): a shifted Cholesky
+    QR pass, then two plain ones, all GEMM / POTRF / TRSM work.  A factor whose
+    Cholesky fails, or whose Q is not orthonormal to 1e-13 per column, is
+    recomputed by Householder QR (rocSOLVER geqrf / orgqr).  The positive-R QR
+    of a full-rank matrix is unique, so both routes give the same factors."""
+    m, n = M.shape
+    if m < n or n == 0:
+        return _householder_pos_qr(M)
+    fro2 = float(torch.sum(M * M))
+    if fro2 == 0.0 or not np.isfinite(fro2):
+        return _householder_pos_qr(M)
+    shift = 11.0 * (m * n + n * (n + 1)) * _EPS * fro2
+    Q = M
+    R = None
+    for step in range(3):
+        L, info = _chol_step(Q, shift if step == 0 else 0.0)
+        if int(info) != 0:
+            return _householder_pos_qr(M)
+        Rk = L.t()
+        Q = torch.linalg.solve_triangular(Rk, Q, upper=True, left=False)
+        R = Rk if R is None else Rk @ R
+    E = Q.t() @ Q
+    E.diagonal().sub_(1.0)
+    if float(torch.linalg.matrix_norm(E)) > 1e-13 * max(n, 1) ** 0.5:
+        return _householder_pos_qr(M)
+    return Q, torch.triu(R)
+
+
 def qr_left(A):
     """A[l,p,r] = Q[l,p,m] R[m,r] with positive diagonal R (qr_compact(positive=true))."""
     l, p, r = A.shape
-    Q, R = torch.linalg.qr(_mat(A, 2))
-    s = torch.sign(torch.diagonal(R))
-    s[s == 0] = 1
-    Q, R = Q * s, s[:, None] * R
+    Q, R = pos_qr(_mat(A, 2))
     return T.as_colmajor(T.unfuse(Q, (l, p, Q.shape[1]))), R
 
 
 def lq_right(A):
     """A[l,p,r] = L[l,m] Q[m,p,r] with Q right-orthonormal."""
     l, p, r = A.shape
-    Qt, Rt = torch.linalg.qr(_mat(A, 1).t())
-    s = torch.sign(torch.diagonal(Rt))
-    s[s == 0] = 1
-    Qt, Rt = Qt * s, s[:, None] * Rt
+    Qt, Rt = pos_qr(_mat(A, 1).t())
     return Rt.t(), T.as_colmajor(T.unfuse(Qt.t(), (Qt.shape[1], p, r)))
 
 

@@ -26,7 +26,7 @@ import torch
 
 from . import tensors as T
 from .derivatives import JordanMPO_AC_Hamiltonian, MPO_AC_Hamiltonian, MPO_C_Hamiltonian, prepare_operator
-from .states import lq_right, qr_left
+from .states import lq_right, pos_qr, qr_left
 from .transfer import transfer_left, transfer_right
 
 
@@ -80,37 +80,74 @@ def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50):
     return x, dict(normres=beta, numops=ops, converged=beta <= tol)
 
 
-def arnoldi_dominant(apply, v0, tol=1e-12, krylovdim=20, maxiter=20):
+def _flat(t):
+    """Column-major tensor -> contiguous 1-D view of its storage (copy otherwise)."""
+    return T.as_colmajor(t).permute(*reversed(range(t.dim()))).reshape(-1)
+
+
+def _unflat(f, shape):
+    """Inverse of _flat: a column-major view of the 1-D tensor ``f``."""
+    return f.reshape(tuple(reversed(shape))).permute(*reversed(range(len(shape))))
+
+
+def arnoldi_dominant(apply, v0, tol=1e-12, krylovdim=20, maxiter=20, relative=False, info=None):
     """Dominant (largest |lambda|) eigenvector of a real map, restarted Arnoldi
-    (KrylovKit.eigsolve(..., :LM) as used by fixedpoint in ortho.jl:307-313)."""
-    v = v0 / _norm(v0)
+    (KrylovKit.eigsolve(..., :LM) as used by fixedpoint in ortho.jl:307-313 and
+    infinite_envs.jl:132-134).  Eager: the Ritz pair is tested after every
+    Krylov step, so a converged start vector costs one apply.  The residual
+    |H[j+1, j] s_j| is absolute, or relative to |lambda| with ``relative``.
+    Ritz values within 1e-6 of the largest modulus count as tied; of those the
+    one with the largest real part is taken (KrylovKit leaves such ties to its
+    sort order), so a transfer map returns its positive fixed point.
+    The basis is one (krylovdim + 1) x n GPU matrix; each step orthogonalises
+    with two classical Gram-Schmidt passes (GEMVs) and reads H back once."""
+    shape = tuple(v0.shape)
+    x = _flat(v0)
+    n = x.numel()
+    Vb = torch.empty((krylovdim + 1, n), dtype=torch.float64, device=x.device)
+    Vb[0] = x / torch.linalg.vector_norm(x)
     lam = 0.0
+    ops = 0
+    res = math.inf
+
+    def done(res, lam, hnext):
+        return res <= (tol * max(abs(lam), 1e-300) if relative else tol) or hnext <= 1e-14
+
     for _ in range(maxiter):
-        V = [v]
         H = np.zeros((krylovdim + 1, krylovdim))
-        k = krylovdim
         for j in range(krylovdim):
-            w = apply(V[j])
-            for _pass in range(2):
-                for i in range(j + 1):
-                    h = float(_dot(V[i], w))
-                    H[i, j] += h
-                    w = w - h * V[i]
-            H[j + 1, j] = float(_norm(w))
-            if H[j + 1, j] <= 1e-14:
-                k = j + 1
+            w = _flat(apply(_unflat(Vb[j], shape)))
+            ops += 1
+            Vj = Vb[: j + 1]
+            h1 = Vj @ w
+            w = w - Vj.t() @ h1
+            h2 = Vj @ w
+            w = w - Vj.t() @ h2
+            hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+            H[: j + 1, j] = hcol[:-1]
+            H[j + 1, j] = hcol[-1]
+            ev, evec = np.linalg.eig(H[: j + 1, : j + 1])
+            # :LM with ties on the largest modulus (peripheral spectrum of a
+            # non-injective MPS, e.g. a complex pair beside the real fixed point)
+            # resolved towards the largest real part
+            peri = np.flatnonzero(np.abs(ev) >= np.abs(ev).max() * (1 - 1e-6))
+            i = int(peri[np.argmax(np.real(ev[peri]))])
+            s = evec[:, i]
+            s = s * np.exp(-1j * np.angle(s[int(np.argmax(np.abs(s)))]))
+            lam = float(np.real(ev[i]))
+            res = abs(H[j + 1, j] * s[-1]) / np.linalg.norm(s)
+            if done(res, lam, H[j + 1, j]):
                 break
-            V.append(w / H[j + 1, j])
-        ev, evec = np.linalg.eig(H[:k, :k])
-        i = int(np.argmax(np.abs(ev)))
-        s = np.real(evec[:, i])
-        lam = float(np.real(ev[i]))
-        v = sum(float(c) * V[j] for j, c in enumerate(s))
-        v = v / _norm(v)
-        res = abs(H[k, k - 1] * s[-1]) if k < krylovdim + 1 and k == krylovdim else 0.0
-        if res <= tol:
+            Vb[j + 1] = w / H[j + 1, j]
+        sr = torch.from_numpy(np.ascontiguousarray(np.real(s))).to(Vb.device)
+        x = Vb[: sr.numel()].t() @ sr
+        x = x / torch.linalg.vector_norm(x)
+        if done(res, lam, H[j + 1, j]):
             break
-    return lam, v
+        Vb[0] = x
+    if info is not None:
+        info.update(numops=ops, normres=float(res))
+    return lam, _unflat(x, shape)
 
 
 # ---------------------------------------------------------------------------
@@ -126,10 +163,7 @@ def _timesc(A, C):
 
 
 def _pos_qr(M):
-    Q, R = torch.linalg.qr(M)
-    s = torch.sign(torch.diagonal(R))
-    s[s == 0] = 1
-    return Q * s, s[:, None] * R
+    return pos_qr(M)
 
 
 def uniform_leftorth(A, C0, tol=1e-14, maxiter=200, eig_miniter=10):

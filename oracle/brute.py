@@ -192,3 +192,43 @@ def tfim_infinite_energy(g):
     from scipy.integrate import quad
     val, _ = quad(lambda k: np.sqrt(1.0 + g * g - 2.0 * g * np.cos(k)), 0.0, np.pi, epsabs=1e-14, epsrel=1e-14)
     return -val / np.pi
+
+
+CATALAN = 0.915965594177219015054603514932384110774
+
+
+def ising2d_log_partition(beta):
+    """Square-lattice Ising model (J = 1, no field), ln Z per site (Onsager,
+    single-integral form): ln(2 cosh 2b) + (1/pi) int_0^{pi/2}
+    ln[(1 + sqrt(1 - k^2 sin^2 t)) / 2] dt with k = 2 sinh 2b / cosh^2 2b."""
+    from scipy.integrate import quad
+    k = 2.0 * np.sinh(2 * beta) / np.cosh(2 * beta) ** 2
+    f = lambda t: np.log(0.5 * (1.0 + np.sqrt(max(1.0 - (k * np.sin(t)) ** 2, 0.0))))
+    val, _ = quad(f, 0.0, np.pi / 2, epsabs=1e-14, epsrel=1e-14, limit=200)
+    return np.log(2.0 * np.cosh(2 * beta)) + val / np.pi
+
+
+def ising2d_log_partition_double(beta):
+    """The same from Onsager's double integral: ln 2 + (1/(8 pi^2)) int int
+    ln[cosh^2 2b - sinh 2b (cos t1 + cos t2)] dt1 dt2 over [0, 2 pi]^2 (an
+    independent form: the inner integral is done in closed form,
+    int_0^{2pi} ln(a - b cos t) dt = 2 pi ln((a + sqrt(a^2 - b^2)) / 2))."""
+    from scipy.integrate import quad
+    c2, s2 = np.cosh(2 * beta) ** 2, np.sinh(2 * beta)
+
+    def inner(t1):
+        a = c2 - s2 * np.cos(t1)
+        return 2 * np.pi * np.log(0.5 * (a + np.sqrt(max(a * a - s2 * s2, 0.0))))
+
+    val, _ = quad(inner, 0.0, 2 * np.pi, epsabs=1e-13, epsrel=1e-13, limit=200)
+    return np.log(2.0) + val / (8 * np.pi ** 2)
+
+
+def ising2d_torus_partition(beta, L):
+    """Brute-force Z of an L x L periodic Ising lattice (2^(L*L) configurations)."""
+    n = L * L
+    cfg = ((np.arange(2 ** n)[:, None] >> np.arange(n)[None, :]) & 1) * 2 - 1
+    s = cfg.reshape(-1, L, L)
+    e = (s * np.roll(s, 1, axis=1)).sum(axis=(1, 2)) + (s * np.roll(s, 1, axis=2)).sum(axis=(1, 2))
+    return float(np.exp(beta * e).sum())
+

@@ -70,3 +70,54 @@ def test_truncated_split_batched_matches_single_and_svd():
                 assert torch.allclose(A @ B, A1 @ B1, atol=1e-10)
                 S2 = truncated_split(M, direction, trunc_dim, trunc_err, method="eigh")[2]
                 assert torch.allclose(S, S2, atol=1e-12)
+
+
+@pytest.mark.parametrize("L", [3, 4])
+def test_classical_ising_mpo_contracts_to_the_torus_partition_function(L):
+    """Tr(T_row^L) of the L x L torus built from the MPO tensor equals the
+    brute-force sum over all 2^(L*L) spin configurations."""
+    from oracle import brute as B
+    beta = 0.37
+    O = M.classical_ising(beta)
+    if L == 3:
+        Trow = np.einsum("pbhq,qcir,rdjp->bcdhij", O, O, O).reshape(8, 8)
+    else:
+        Trow = np.einsum("pbhq,qcir,rdjs,setp->bcdehijt", O, O, O, O).reshape(16, 16)
+    Z = np.trace(np.linalg.matrix_power(Trow, L))
+    ref = B.ising2d_torus_partition(beta, L)
+    assert abs(Z - ref) < 1e-11 * ref
+
+
+@pytest.mark.parametrize("cond", [1.0, 1e4, 1e8, 1e12])
+def test_pos_qr_matches_householder(cond):
+    """Shifted Cholesky QR3 gives the unique positive-diagonal QR: the same
+    factors as Householder QR with signs fixed, for condition numbers up to
+    1e12, with Q orthonormal to rounding."""
+    import torch
+    from mpskit_amd.states import _householder_pos_qr, pos_qr
+    rng = np.random.default_rng(5)
+    m, n = 96, 48
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = torch.from_numpy(U @ np.diag(np.logspace(0, -np.log10(cond), n)) @ V.T)
+    Q, R = pos_qr(A)
+    Qh, Rh = _householder_pos_qr(A)
+    assert torch.all(torch.diagonal(R) > 0)
+    assert float(torch.linalg.matrix_norm(Q.t() @ Q - torch.eye(n, dtype=torch.float64))) < 1e-13
+    assert float(torch.linalg.matrix_norm(Q @ R - A)) < 1e-14 * float(torch.linalg.matrix_norm(A))
+    assert float(torch.linalg.matrix_norm(R - Rh)) < 1e-13 * cond ** 0.5
+    if cond <= 1e4:
+        assert float(torch.linalg.matrix_norm(Q - Qh)) < 1e-10
+
+
+def test_pos_qr_rank_deficient_and_wide_fall_back():
+    import torch
+    from mpskit_amd.states import pos_qr
+    rng = np.random.default_rng(6)
+    A = torch.from_numpy(rng.standard_normal((40, 10)) @ rng.standard_normal((10, 20)))   # rank 10
+    Q, R = pos_qr(A)
+    assert float(torch.linalg.matrix_norm(Q @ R - A)) < 1e-12 * float(torch.linalg.matrix_norm(A))
+    assert float(torch.linalg.matrix_norm(Q.t() @ Q - torch.eye(20, dtype=torch.float64))) < 1e-12
+    W = torch.from_numpy(rng.standard_normal((5, 9)))
+    Q, R = pos_qr(W)
+    assert Q.shape == (5, 5) and float(torch.linalg.matrix_norm(Q @ R - W)) < 1e-13

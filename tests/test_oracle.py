@@ -208,3 +208,17 @@ def test_tfim_infinite_closed_form_vs_open_chains():
     for g in (0.3, 0.5, 1.5):
         bulk = (B.tfim_open_energy(800, g) - B.tfim_open_energy(400, g)) / 400
         assert abs(bulk - B.tfim_infinite_energy(g)) < 1e-8, g
+
+
+def test_onsager_closed_forms_agree_and_pin_the_critical_value():
+    """The single- and double-integral forms of Onsager's ln Z per site agree; at
+    beta_c = ln(1 + sqrt 2) / 2 the value is ln sqrt 2 + 2 G / pi, which is the
+    2.5337 per site the reference's leading_boundary test checks
+    (test/algorithms/groundstate.jl:481); beta -> 0 gives ln 2."""
+    for beta in (0.05, 0.3, 0.4, 0.5, 0.9):
+        assert abs(B.ising2d_log_partition(beta) - B.ising2d_log_partition_double(beta)) < 1e-12
+    bc = np.log(1 + np.sqrt(2)) / 2
+    crit = B.ising2d_log_partition(bc)
+    assert abs(crit - (np.log(np.sqrt(2)) + 2 * B.CATALAN / np.pi)) < 1e-12
+    assert abs(np.exp(crit) - 2.5337) < 1e-3
+    assert abs(B.ising2d_log_partition(1e-9) - np.log(2)) < 1e-12
