@@ -42,44 +42,6 @@ def _norm(a):
 # small Krylov solvers over GPU vectors
 # ---------------------------------------------------------------------------
 
-def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50):
-    """Solve apply(x) = b (KrylovKit.linsolve with GMRES, used at infinite_envs.jl:236-324).
-    Returns (x, info) with info = dict(normres, numops, converged)."""
-    x = x0.clone()
-    r = b - apply(x)
-    beta = float(_norm(r))
-    ops = 1
-    bnorm = max(float(_norm(b)), 1e-300)
-    for _ in range(maxiter):
-        if beta <= tol:
-            return x, dict(normres=beta, numops=ops, converged=True)
-        V = [r / beta]
-        H = np.zeros((krylovdim + 1, krylovdim))
-        y = None
-        for j in range(krylovdim):
-            w = apply(V[j])
-            ops += 1
-            for _pass in range(2):
-                for i in range(j + 1):
-                    h = float(_dot(V[i], w))
-                    H[i, j] += h
-                    w = w - h * V[i]
-            H[j + 1, j] = float(_norm(w))
-            e1 = np.zeros(j + 2)
-            e1[0] = beta
-            y, *_ = np.linalg.lstsq(H[: j + 2, : j + 1], e1, rcond=None)
-            res = float(np.linalg.norm(H[: j + 2, : j + 1] @ y - e1))
-            if res <= tol or H[j + 1, j] <= 1e-14 * bnorm:
-                break
-            V.append(w / H[j + 1, j])
-        for i, c in enumerate(y):
-            x = x + float(c) * V[i]
-        r = b - apply(x)
-        ops += 1
-        beta = float(_norm(r))
-    return x, dict(normres=beta, numops=ops, converged=beta <= tol)
-
-
 def _flat(t):
     """Column-major tensor -> contiguous 1-D view of its storage (copy otherwise)."""
     return T.as_colmajor(t).permute(*reversed(range(t.dim()))).reshape(-1)
@@ -88,6 +50,51 @@ def _flat(t):
 def _unflat(f, shape):
     """Inverse of _flat: a column-major view of the 1-D tensor ``f``."""
     return f.reshape(tuple(reversed(shape))).permute(*reversed(range(len(shape))))
+
+
+def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50):
+    """Solve apply(x) = b (KrylovKit.linsolve with GMRES, used at infinite_envs.jl:236-324).
+    Returns (x, info) with info = dict(normres, numops, converged).  The Krylov
+    basis is one (krylovdim + 1) x n GPU matrix, orthogonalised by two classical
+    Gram-Schmidt passes per step with one readback of the Hessenberg column."""
+    shape = tuple(b.shape)
+    bf = _flat(b)
+    x = _flat(x0).clone()
+    r = bf - _flat(apply(_unflat(x, shape)))
+    beta = float(torch.linalg.vector_norm(r))
+    ops = 1
+    bnorm = max(float(torch.linalg.vector_norm(bf)), 1e-300)
+    Vb = torch.empty((krylovdim + 1, bf.numel()), dtype=torch.float64, device=bf.device)
+    for _ in range(maxiter):
+        if beta <= tol:
+            return _unflat(x, shape), dict(normres=beta, numops=ops, converged=True)
+        Vb[0] = r / beta
+        H = np.zeros((krylovdim + 1, krylovdim))
+        y = None
+        for j in range(krylovdim):
+            w = _flat(apply(_unflat(Vb[j], shape)))
+            ops += 1
+            Vj = Vb[: j + 1]
+            h1 = Vj @ w
+            w = w - Vj.t() @ h1
+            h2 = Vj @ w
+            w = w - Vj.t() @ h2
+            hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+            H[: j + 1, j] = hcol[:-1]
+            H[j + 1, j] = hcol[-1]
+            e1 = np.zeros(j + 2)
+            e1[0] = beta
+            y, *_ = np.linalg.lstsq(H[: j + 2, : j + 1], e1, rcond=None)
+            res = float(np.linalg.norm(H[: j + 2, : j + 1] @ y - e1))
+            if res <= tol or H[j + 1, j] <= 1e-14 * bnorm:
+                break
+            Vb[j + 1] = w / H[j + 1, j]
+        yt = torch.from_numpy(np.ascontiguousarray(y)).to(Vb.device)
+        x = x + Vb[: yt.numel()].t() @ yt
+        r = bf - _flat(apply(_unflat(x, shape)))
+        ops += 1
+        beta = float(torch.linalg.vector_norm(r))
+    return _unflat(x, shape), dict(normres=beta, numops=ops, converged=beta <= tol)
 
 
 def arnoldi_dominant(apply, v0, tol=1e-12, krylovdim=20, maxiter=20, relative=False, info=None):
