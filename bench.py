@@ -27,6 +27,7 @@ Other workloads print their own JSON line:
                          (charge-block grouped GEMMs, captured Krylov steps)
     --workload vumps     VUMPS on the J1-J2 chain, cell 2, D = 1024
     --workload boundary  leading_boundary (VUMPS) of the 2D Ising transfer MPO, D = 1024
+    --workload tdvp      imaginary-time TDVP2 on the Heisenberg chain, L = 24, D = 512
 """
 from __future__ import annotations
 
@@ -499,6 +500,43 @@ def vumps_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def tdvp_workload(args):
+    """Imaginary-time TDVP2 on the Heisenberg chain (L = 24, bonds up to
+    --tdvp-D, truncation at --tdvp-D), dt = 0.05, --vumps-iters steps from one
+    random normalised state (seed 0), float64: seconds per time step (steps 2 on),
+    applies per step and the energy per site after the steps."""
+    import torch
+    import mpskit_amd as mk
+    from mpskit_amd import tdvp as Td
+    L, D, steps, dt = 24, args.tdvp_D, args.vumps_iters, 0.05
+    H = mk.FiniteMPOHamiltonian(mk.models.xxz(L, 1.0))
+    psi = mk.FiniteMPS.random(L, 2, D, seed=0)
+    envs = mk.environments(psi, H)
+    times, ops = [], []
+    print(f"# tdvp2 L={L} D={D}", file=sys.stderr, flush=True)
+    for k in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        psi, envs, st = Td.timestep(psi, H, dt, Td.TDVP2(trunc_dim=D), envs, normalize=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        ops.append(st["numops"])
+        print(f"# step {k + 1}: {times[-1]:.3f} s, {st['numops']} applies", file=sys.stderr, flush=True)
+    E = mk.expectation_value(psi, H, envs) / L
+    per = times[1:] or times
+    out = {
+        "metric": "imaginary-time TDVP2 time per step, Heisenberg chain L=24",
+        "value": sum(per) / len(per), "unit": "s/step", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": sum(per) / len(per) * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"TDVP2 imaginary time, Heisenberg L={L}, D<={D}, dt={dt}, random start seed 0, "
+                               f"{steps} steps"},
+        "applies_per_step": sum(ops[1:] or ops) / len(ops[1:] or ops), "energy_per_site": E,
+        "bond_dims": psi.bond_dims(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def boundary_workload(args):
     """leading_boundary of the square-lattice Ising transfer MPO (beta = 0.42,
     disordered side of beta_c = 0.4407, unit cell 1, w = d = 2) with VUMPS at bond
@@ -546,12 +584,13 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
                     help="time one precision only (for profiler runs); prints no JSON line")
     ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048")
-    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg", "idmrg2", "boundary"], default="apply")
+    ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg", "idmrg2", "boundary", "tdvp"], default="apply")
     ap.add_argument("--vumps-D", type=int, default=1024)
     ap.add_argument("--u1-sweeps", type=int, default=2)
     ap.add_argument("--vumps-iters", type=int, default=6)
     ap.add_argument("--idmrg2-D", type=int, default=256, help="kept states of the IDMRG2 workload")
     ap.add_argument("--boundary-D", type=int, default=1024, help="bond dimension of the boundary workload")
+    ap.add_argument("--tdvp-D", type=int, default=512, help="maximal bond dimension of the TDVP2 workload")
     ap.add_argument("--L", type=int, default=48, help="chain length of the batched workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device-to-device five times (HBM counter calibration) and exit")
@@ -594,6 +633,9 @@ def main():
         return
     if args.workload == "boundary":
         boundary_workload(args)
+        return
+    if args.workload == "tdvp":
+        tdvp_workload(args)
         return
 
     D, d = args.D, 2
