@@ -260,7 +260,9 @@ def local_update(pos, direction, psi, H, alg: DMRG2, envs, eps_global, eps_trunc
     ac2 = psi.AC2(pos)
     y = Heff(ac2)
     y = y / torch.linalg.vector_norm(y)
-    ALp = psi.AL[pos] if psi.center == pos + 1 else qr_left(psi.AC)[0]
+    # the center of a sweep is numerically rank deficient (Schmidt values below
+    # roundoff), where Cholesky QR cannot succeed: Householder directly
+    ALp = psi.AL[pos] if psi.center == pos + 1 else qr_left(psi.AC, householder=True)[0]
     eps_local = float(torch.linalg.vector_norm(project_complement(y, ALp)))
     tol, krylovdim, maxiter = alg.alg_eigsolve.adapt_solver(decay_rate=decay_rate, g_local=eps_local,
                                                             g_global=eps_global, eps_trunc=eps_trunc)

@@ -92,10 +92,11 @@ def pos_qr(M):
     return Q, torch.triu(R)
 
 
-def qr_left(A):
-    """A[l,p,r] = Q[l,p,m] R[m,r] with positive diagonal R (qr_compact(positive=true))."""
+def qr_left(A, householder=False):
+    """A[l,p,r] = Q[l,p,m] R[m,r] with positive diagonal R (qr_compact(positive=true));
+    ``householder`` skips the Cholesky QR attempt (for inputs known to be rank deficient)."""
     l, p, r = A.shape
-    Q, R = pos_qr(_mat(A, 2))
+    Q, R = (_householder_pos_qr if householder else pos_qr)(_mat(A, 2))
     return T.as_colmajor(T.unfuse(Q, (l, p, Q.shape[1]))), R
 
 
