@@ -24,7 +24,7 @@ __all__ = [
     "SingleTransferMatrix", "TransferMatrix", "regularize", "regularize_", "transfer_left", "transfer_right",
 ]
 from .algorithms import (
-    DMRG2, AdaptiveKrylov, Davidson, Lanczos, expectation_value, find_groundstate, find_groundstate_batched,
+    DMRG, DMRG2, AdaptiveKrylov, Davidson, Lanczos, expectation_value, find_groundstate, find_groundstate_batched,
 )
 from .learned import MLPPreconditioner, davidson
 from .u1 import U1Bond, U1FiniteMPS, find_groundstate_u1, set_charge_blocks
@@ -33,5 +33,5 @@ from .states import FiniteMPOHamiltonian, FiniteMPS
 
 __all__ += ["Davidson", "MLPPreconditioner", "davidson", "find_groundstate_batched", "U1Bond", "U1FiniteMPS",
             "find_groundstate_u1", "set_charge_blocks"]
-__all__ += ["DMRG2", "AdaptiveKrylov", "Lanczos", "expectation_value", "find_groundstate", "FiniteEnvironments",
+__all__ += ["DMRG", "DMRG2", "AdaptiveKrylov", "Lanczos", "expectation_value", "find_groundstate", "FiniteEnvironments",
             "environments", "FiniteMPOHamiltonian", "FiniteMPS"]

@@ -21,7 +21,7 @@ import torch
 from . import tensors as T
 from .derivatives import JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, prepare_operator
 from .environments import FiniteEnvironments
-from .states import FiniteMPS, qr_left
+from .states import FiniteMPS, lq_right, qr_left
 
 
 @dataclass
@@ -285,8 +285,88 @@ def local_update(pos, direction, psi, H, alg: DMRG2, envs, eps_global, eps_trunc
     return eps_local, eps_trunc, decay_rate
 
 
-def find_groundstate(psi: FiniteMPS, H, alg: DMRG2, envs: FiniteEnvironments | None = None):
-    """``find_groundstate(psi, H, DMRG2(...))`` (dmrg.jl:278-356): returns (psi, envs, eps, history)."""
+@dataclass
+class DMRG:
+    """Single-site DMRG (dmrg.jl:62-170): one-site Jordan dAC eigensolves and a
+    QR / LQ gauge step that moves the center (no truncation, no expansion, so
+    the bond dimensions of the start state are kept)."""
+    tol: float = 1e-10
+    maxiter: int = 100
+    verbosity: int = 1
+    alg_eigsolve: object = field(default_factory=AdaptiveKrylov)
+    precision: str = "fp64"
+    exact_sweeps: bool = False
+
+
+def local_update1(pos, direction, psi, H, alg: DMRG, envs, eps_global, decay_rate, stats):
+    """``local_update!(::DMRG)`` (dmrg.jl:124-158): Galerkin error of the site,
+    dAC eigensolve, gauge with ``normalize = true``."""
+    psi.move_center(pos)
+    precs = ("fp64",) if alg.precision == "fp64" else ("fp64", alg.precision)
+    Heff = prepare_operator(JordanMPO_AC_Hamiltonian(envs.leftenv(pos, psi), H[pos], envs.rightenv(pos, psi)),
+                            precisions=precs)
+    y = Heff(psi.AC)
+    y = y / torch.linalg.vector_norm(y)
+    # calc_galerkin(site, ...) (toolbox.jl:63-70); the center of a converging
+    # sweep is numerically rank deficient, so Householder QR directly
+    eps_local = float(torch.linalg.vector_norm(project_complement(y, qr_left(psi.AC, householder=True)[0])))
+    tol, krylovdim, maxiter = alg.alg_eigsolve.adapt_solver(decay_rate=decay_rate, g_local=eps_local,
+                                                            g_global=eps_global, eps_trunc=0.0)
+    krylovdim = max(2, min(krylovdim, psi.AC.numel()))
+    _, ac, info = Heff.eigsolve(psi.AC, krylovdim=krylovdim, maxiter=maxiter, tol=tol, precision=alg.precision)
+    Heff.free()
+    stats["numops"] += info["numops"] + 1
+    ac = T.as_colmajor(ac)
+    if direction == "right":
+        AL, C = qr_left(ac, householder=True)
+        psi.AL[pos] = AL
+        nxt = torch.einsum("lm,mpr->lpr", C, psi.AR[pos + 1])
+        psi.AR[pos + 1] = None
+        psi.center = pos + 1
+    else:
+        C, AR = lq_right(ac, householder=True)
+        psi.AR[pos] = AR
+        nxt = torch.einsum("lpm,mr->lpr", psi.AL[pos - 1], C)
+        psi.AL[pos - 1] = None
+        psi.center = pos - 1
+    psi.AC = T.as_colmajor(nxt / torch.linalg.vector_norm(nxt))
+    numops = max(1, info["numops"])
+    decay = (info["normres"] / eps_local) ** (1.0 / numops) if eps_local > 0 else 0.0
+    return eps_local, min(max(decay, 1e-3), 0.999)
+
+
+def _find_groundstate_dmrg1(psi: FiniteMPS, H, alg: DMRG, envs: FiniteEnvironments | None):
+    L = psi.L
+    envs = envs or FiniteEnvironments(psi, H)
+    eps_locals = [1.0] * L
+    decay_rates = [0.0] * L
+    eps_global = 1.0
+    history = []
+    for it in range(1, alg.maxiter + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = {"numops": 0}
+        # _sweep_ranges(::DMRG): sites 1..L-1 to the right, L..2 to the left (1-based)
+        for pos, direction in [(p, "right") for p in range(L - 1)] + [(p, "left") for p in range(L - 1, 0, -1)]:
+            eps_locals[pos], decay_rates[pos] = local_update1(pos, direction, psi, H, alg, envs, eps_global,
+                                                              decay_rates[pos], stats)
+            eps_global = max(eps_locals)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        E = expectation_value(psi, H, envs)
+        history.append(dict(iter=it, energy=E, eps=eps_global, trunc=0.0, time=dt, numops=stats["numops"]))
+        if alg.verbosity >= 2:
+            print(f"DMRG {it:3d}: E = {E:.15f}  err = {eps_global:.3e}  time = {dt:.3f}s", flush=True)
+        if not alg.exact_sweeps and eps_global <= alg.tol:
+            break
+    return psi, envs, eps_global, history
+
+
+def find_groundstate(psi: FiniteMPS, H, alg, envs: FiniteEnvironments | None = None):
+    """``find_groundstate(psi, H, DMRG2(...) | DMRG(...))`` (dmrg.jl:278-356):
+    returns (psi, envs, eps, history)."""
+    if isinstance(alg, DMRG):
+        return _find_groundstate_dmrg1(psi, H, alg, envs)
     L = psi.L
     envs = envs or FiniteEnvironments(psi, H)
     n = L - 1

@@ -100,10 +100,10 @@ def qr_left(A, householder=False):
     return T.as_colmajor(T.unfuse(Q, (l, p, Q.shape[1]))), R
 
 
-def lq_right(A):
-    """A[l,p,r] = L[l,m] Q[m,p,r] with Q right-orthonormal."""
+def lq_right(A, householder=False):
+    """A[l,p,r] = L[l,m] Q[m,p,r] with Q right-orthonormal; ``householder`` as in qr_left."""
     l, p, r = A.shape
-    Qt, Rt = pos_qr(_mat(A, 1).t())
+    Qt, Rt = (_householder_pos_qr if householder else pos_qr)(_mat(A, 1).t())
     return Rt.t(), T.as_colmajor(T.unfuse(Qt.t(), (Qt.shape[1], p, r)))
 
 

@@ -101,3 +101,27 @@ def test_eigh_gauge_matches_svd_gauge(mk):
     _, _, _, hist = mk.find_groundstate(psi, H, alg)
     assert abs(hist[-1]["energy"] - B.tfim_open_energy(L, g_)) / L < 1e-10
 
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_single_site_dmrg_tfim_full_bond(mk, precision):
+    """DMRG (one site, dmrg.jl:62-170) keeps the start state's bonds: from a
+    random state with full bond dimensions it reaches the open-TFIM closed form."""
+    L, g = 12, 1.0
+    H = mk.FiniteMPOHamiltonian(mk.models.tfim(L, g))
+    psi = mk.FiniteMPS.random(L, 2, 64, seed=3)
+    print("dmrg1 tfim L=12", precision, flush=True)
+    psi, envs, eps, hist = mk.find_groundstate(psi, H, mk.DMRG(tol=1e-10, maxiter=30, precision=precision,
+                                                               verbosity=2))
+    assert abs(hist[-1]["energy"] - B.tfim_open_energy(L, g)) / L < 1e-10, hist[-1]
+    assert psi.bond_dims() == [min(2 ** k, 2 ** (L - k), 64) for k in range(L + 1)]
+
+
+def test_single_site_dmrg_xxz_vs_exact_diagonalisation(mk):
+    L, delta = 10, 0.7
+    H = mk.FiniteMPOHamiltonian(mk.models.xxz(L, delta))
+    psi = mk.FiniteMPS.random(L, 2, 32, seed=4)
+    print("dmrg1 xxz L=10", flush=True)
+    psi, envs, eps, hist = mk.find_groundstate(psi, H, mk.DMRG(tol=1e-10, maxiter=30, verbosity=2))
+    e_ed = np.linalg.eigvalsh(B.dense_xxz(L, delta).toarray())[0]
+    assert abs(hist[-1]["energy"] - e_ed) / L < 1e-10
