@@ -1,0 +1,100 @@
+"""Bond expansion of an InfiniteMPS (changebonds with OptimalExpand) on the HIP path.
+
+Mirrors (paths relative to the MPSKit.jl tree):
+  OptimalExpand  src/algorithms/changebonds/optimalexpand.jl:1-110 (Zauner-Stauber
+                 et al., PRB 97 (2018)): per bond, the two-site update dAC2(AC AR)
+                 projected onto the null spaces of AL and AR, its dominant
+                 ``trunc_dim`` singular directions added to the bond
+  _expand!       src/algorithms/changebonds/changebonds.jl:59-84: AL -> [AL expansion;
+                 0 0], AR -> [AR 0; expansion 0], C -> [C 0; 0 0] (the state is unchanged)
+
+The two-site update is the HIP library's dAC2 apply: the Jordan operator for an
+InfiniteHamiltonian (environments of vumps.InfiniteEnvironments), the generic
+one for an InfiniteMPO (boundary.BoundaryEnvironments).  Null spaces and the
+small SVD are float64 torch.linalg (rocSOLVER) calls.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from . import tensors as T
+from .derivatives import JordanMPO_AC2_Hamiltonian, MPO_AC2_Hamiltonian, prepare_operator
+from .vumps import InfiniteMPS
+
+
+@dataclass
+class OptimalExpand:
+    """OptimalExpand(; trunc = truncrank(trunc_dim)): at most ``trunc_dim``
+    directions are added to every bond."""
+    trunc_dim: int = 1
+
+
+def _left_null(AL):
+    """Orthonormal complement of the columns of AL as an (l p) x r isometry."""
+    M = T.fuse(AL, 2)
+    m, r = M.shape
+    Q, _ = torch.linalg.qr(M, mode="complete")
+    return Q[:, r:]
+
+
+def _right_null(AR):
+    """Orthonormal complement of the rows of AR as an l x (p r) co-isometry, as rows."""
+    M = T.fuse(AR, 1)
+    l, n = M.shape
+    Q, _ = torch.linalg.qr(M.t(), mode="complete")
+    return Q[:, l:].t()
+
+
+def changebonds(psi: InfiniteMPS, H, alg: OptimalExpand, envs):
+    """``changebonds(psi, H, OptimalExpand(...), envs)`` for an InfiniteHamiltonian
+    (vumps.InfiniteHamiltonian) or an InfiniteMPO (boundary.InfiniteMPO); returns
+    the expanded state (environments must be recomputed by the caller)."""
+    N = psi.N
+    jordan = hasattr(H, "W")
+    ALx, ARx = [None] * N, [None] * N          # ALx[i]: added columns of AL[i]; ARx[i]: added rows of AR[i]
+    for i in range(N):
+        j = (i + 1) % N
+        ac2 = T.as_colmajor(torch.einsum("fhm,mjr->fhrj", psi.AC[i], psi.AR[j]))
+        if jordan:
+            desc = JordanMPO_AC2_Hamiltonian(envs.GLs[i], H.W[i], H.W[j], envs.GRs[j])
+        else:
+            desc = MPO_AC2_Hamiltonian(envs.GLs[i], H.O[i], H.O[j], envs.GRs[j])
+        op = prepare_operator(desc)
+        y = op(ac2)
+        op.free()
+        # rows (f, h), columns (j, r) with j fastest, as the fused AR[j]
+        Y = T.fuse(T.as_colmajor(y.permute(0, 1, 3, 2)), 2)
+        VL, VR = _left_null(psi.AL[i]), _right_null(psi.AR[j])
+        k = min(alg.trunc_dim, VL.shape[1], VR.shape[0])
+        Dl, d, Dr = psi.AL[i].shape
+        d2, Dr2 = psi.AR[j].shape[1], psi.AR[j].shape[2]
+        if k == 0:
+            ALx[i] = T.zeros((Dl, d, 0))
+            ARx[j] = T.zeros((0, d2, Dr2))
+            continue
+        X = VL.t() @ Y @ VR.t()
+        nrm = float(torch.linalg.matrix_norm(X))
+        if nrm > torch.finfo(torch.float64).eps ** 0.75:
+            X = X / nrm
+        U, _, Vh = torch.linalg.svd(X, full_matrices=False)
+        ALx[i] = T.as_colmajor(T.unfuse(VL @ U[:, :k], (Dl, d, k)))
+        ARx[j] = T.as_colmajor(T.unfuse(Vh[:k] @ VR, (k, d2, Dr2)))
+    # _expand! (changebonds.jl:59-84)
+    AL, AR, C = [None] * N, [None] * N, [None] * N
+    for i in range(N):
+        kl, kr = ALx[i - 1].shape[2], ALx[i].shape[2]     # added to the left / right bond of site i
+        Dl, d, Dr = psi.AL[i].shape
+        a = T.zeros((Dl + kl, d, Dr + kr))
+        a[:Dl, :, :Dr] = psi.AL[i]
+        a[:Dl, :, Dr:] = ALx[i]
+        AL[i] = a
+        b = T.zeros((Dl + kl, d, Dr + kr))
+        b[:Dl, :, :Dr] = psi.AR[i]
+        b[Dl:, :, :Dr] = ARx[i]
+        AR[i] = b
+        c = T.zeros((Dr + kr, Dr + kr))
+        c[:Dr, :Dr] = psi.C[i]
+        C[i] = c
+    return InfiniteMPS(AL, AR, C)

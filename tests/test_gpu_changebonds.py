@@ -1,0 +1,60 @@
+"""changebonds with OptimalExpand on infinite MPS (HIP dAC2 applies).
+
+The expansion must leave the state unchanged (same energy / partition function
+per site right after it) and grow every bond by the requested amount; after
+re-converging, the larger bond dimension reaches the closed forms: the
+reference's own leading_boundary check (2D Ising at beta_c, D = 10 -> 13,
+2.5337 per site to 1e-3, test/algorithms/groundstate.jl:456-483) and the TFIM
+energy per site (Pfeuty) with VUMPS.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import brute as B  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def mk():
+    import torch
+    assert torch.cuda.is_available()
+    import mpskit_amd
+    mpskit_amd.load()
+    from mpskit_amd import boundary, changebonds, vumps
+    return mpskit_amd, boundary, changebonds, vumps
+
+
+def test_optimal_expand_ising_boundary(mk):
+    m, Bd, Cb, V = mk
+    beta = float(np.log(1 + np.sqrt(2)) / 2)
+    O = Bd.InfiniteMPO([m.models.classical_ising(beta)])
+    psi = V.InfiniteMPS.random(1, 2, 10, seed=0)
+    print("boundary D=10 -> 13", flush=True)
+    psi, envs, _, hist = Bd.leading_boundary(psi, O, Bd.BoundaryVUMPS(tol=1e-4, maxiter=100, verbosity=1))
+    before = Bd.expectation_value_mpo(psi, O, envs)
+    psi = Cb.changebonds(psi, O, Cb.OptimalExpand(trunc_dim=3), envs)
+    assert tuple(psi.AL[0].shape) == (13, 2, 13)
+    after = Bd.expectation_value_mpo(psi, O, Bd.BoundaryEnvironments(psi, O))
+    assert abs(after - before) < 1e-8 * abs(before)
+    psi, envs, _, hist = Bd.leading_boundary(psi, O, Bd.BoundaryVUMPS(tol=1e-4, maxiter=100, verbosity=1))
+    assert tuple(psi.AL[0].shape) == (13, 2, 13)
+    assert abs(hist[-1]["value"] - 2.5337) < 1e-3
+
+
+@pytest.mark.parametrize("cell", [1, 2])
+def test_optimal_expand_vumps_tfim(mk, cell):
+    m, Bd, Cb, V = mk
+    g = 0.5
+    H = V.InfiniteHamiltonian([m.models.tfim_bulk(g)] * cell)
+    psi = V.InfiniteMPS.random(cell, 2, 6, seed=1)
+    print("vumps tfim D=6 -> 12, cell", cell, flush=True)
+    psi, envs, _, hist = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=1e-8, maxiter=60, verbosity=1))
+    e_before = V.energy_per_site(psi, H, envs)
+    # the growth is capped by the local complement, d D - D = 6 here
+    psi = Cb.changebonds(psi, H, Cb.OptimalExpand(trunc_dim=8), envs)
+    assert all(tuple(a.shape) == (12, 2, 12) for a in psi.AL)
+    e_after = V.energy_per_site(psi, H, V.InfiniteEnvironments(psi, H, tol=1e-12))
+    assert abs(e_after - e_before) < 1e-9
+    psi, envs, eps, hist = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=1e-11, maxiter=60, verbosity=1))
+    assert abs(hist[-1]["energy"] - B.tfim_infinite_energy(g)) < 1e-10
