@@ -1,4 +1,4 @@
-"""Bond expansion of an InfiniteMPS (changebonds with OptimalExpand) on the HIP path.
+"""Bond expansion and truncation (changebonds with OptimalExpand / SvdCut) on the HIP path.
 
 Mirrors (paths relative to the MPSKit.jl tree):
   OptimalExpand  src/algorithms/changebonds/optimalexpand.jl:1-110 (Zauner-Stauber
@@ -7,6 +7,9 @@ Mirrors (paths relative to the MPSKit.jl tree):
                  ``trunc_dim`` singular directions added to the bond
   _expand!       src/algorithms/changebonds/changebonds.jl:59-84: AL -> [AL expansion;
                  0 0], AR -> [AR 0; expansion 0], C -> [C 0; 0 0] (the state is unchanged)
+  SvdCut         src/algorithms/changebonds/svdcut.jl: truncated SVD of every bond
+                 matrix C in the mixed-canonical gauge, right to left for a finite
+                 MPS; for an infinite MPS AL[i] U, U^T AL[i+1] and a new gauge fix
 
 The two-site update is the HIP library's dAC2 apply: the Jordan operator for an
 InfiniteHamiltonian (environments of vumps.InfiniteEnvironments), the generic
@@ -21,6 +24,7 @@ import torch
 
 from . import tensors as T
 from .derivatives import JordanMPO_AC2_Hamiltonian, MPO_AC2_Hamiltonian, prepare_operator
+from .states import FiniteMPS, lq_right
 from .vumps import InfiniteMPS
 
 
@@ -98,3 +102,54 @@ def changebonds(psi: InfiniteMPS, H, alg: OptimalExpand, envs):
         c[:Dr, :Dr] = psi.C[i]
         C[i] = c
     return InfiniteMPS(AL, AR, C)
+
+
+@dataclass
+class SvdCut:
+    """SvdCut(; trunc = truncrank(trunc_dim) & truncerror(trunc_err)): keep at most
+    ``trunc_dim`` singular values per bond and drop a tail of 2-norm <= ``trunc_err``."""
+    trunc_dim: int = 64
+    trunc_err: float = 0.0
+
+
+def _keep(S, alg: SvdCut):
+    k = min(alg.trunc_dim, int(S.shape[0]))
+    if alg.trunc_err > 0.0:
+        tail = torch.sqrt(torch.flip(torch.cumsum(torch.flip(S * S, [0]), 0), [0]))
+        k = max(1, min(k, int((tail > alg.trunc_err).sum().item())))
+    return k
+
+
+def svdcut_finite(psi: FiniteMPS, alg: SvdCut) -> FiniteMPS:
+    """changebonds!(psi::FiniteMPS, SvdCut(...)) (svdcut.jl:22-32), in place:
+    bonds L-2 .. 0 in turn, each bond matrix from an LQ of the center."""
+    for b in range(psi.L - 2, -1, -1):
+        psi.move_center(b + 1)
+        C, AR = lq_right(psi.AC, householder=True)
+        U, S, Vh = torch.linalg.svd(C, full_matrices=False)
+        k = _keep(S, alg)
+        psi.AR[b + 1] = T.as_colmajor(torch.einsum("km,mpr->kpr", Vh[:k], AR))
+        ac = torch.einsum("lpm,mk->lpk", psi.AL[b], U[:, :k] * S[:k])
+        psi.AL[b] = None
+        psi.AC = T.as_colmajor(ac / torch.linalg.vector_norm(ac))
+        psi.center = b
+    return psi
+
+
+def svdcut_infinite(psi: InfiniteMPS, alg: SvdCut) -> InfiniteMPS:
+    """changebonds(psi::InfiniteMPS, SvdCut(...)) (svdcut.jl:80-100): truncate each
+    bond in the singular basis of its C, then gauge-fix the new tensors."""
+    N = psi.N
+    A = [a.clone() for a in psi.AL]
+    for i in range(N):
+        U, S, _ = torch.linalg.svd(psi.C[i], full_matrices=False)
+        Uk = U[:, : _keep(S, alg)]
+        A[i] = T.as_colmajor(torch.einsum("lpm,mk->lpk", A[i], Uk))
+        j = (i + 1) % N
+        A[j] = T.as_colmajor(torch.einsum("mk,mpr->kpr", Uk, A[j]))
+    return InfiniteMPS.from_tensors(A)
+
+
+def _copy_finite(psi: FiniteMPS) -> FiniteMPS:
+    return FiniteMPS([T.as_colmajor(psi.tensor_at(j).clone()) for j in range(psi.L)])
+

@@ -58,3 +58,38 @@ def test_optimal_expand_vumps_tfim(mk, cell):
     assert abs(e_after - e_before) < 1e-9
     psi, envs, eps, hist = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=1e-11, maxiter=60, verbosity=1))
     assert abs(hist[-1]["energy"] - B.tfim_infinite_energy(g)) < 1e-10
+
+
+def test_svdcut_finite_matches_schmidt_truncation(mk):
+    """Cutting a converged L = 12 TFIM ground state to D = 8 keeps the state to
+    its discarded weight; a cut above the current bonds changes nothing."""
+    import torch
+    m, Bd, Cb, V = mk
+    L, g = 12, 1.0
+    H = m.FiniteMPOHamiltonian(m.models.tfim(L, g))
+    psi = m.FiniteMPS.random(L, 2, 64, seed=2)
+    print("svdcut finite L=12", flush=True)
+    psi, _, _, _ = m.find_groundstate(psi, H, m.DMRG2(tol=1e-12, maxiter=8, trunc_dim=64, verbosity=0))
+    v0 = psi.to_dense().cpu().numpy()
+    same = Cb.svdcut_finite(Cb._copy_finite(psi), Cb.SvdCut(trunc_dim=64))
+    assert abs(abs(np.dot(same.to_dense().cpu().numpy(), v0)) - 1.0) < 1e-12
+    cut = Cb.svdcut_finite(Cb._copy_finite(psi), Cb.SvdCut(trunc_dim=8))
+    assert max(cut.bond_dims()) == 8
+    v = cut.to_dense().cpu().numpy()
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-12
+    assert 1.0 - abs(np.dot(v, v0)) < 1e-4
+
+
+def test_svdcut_infinite_tfim(mk):
+    m, Bd, Cb, V = mk
+    g = 0.5
+    H = V.InfiniteHamiltonian([m.models.tfim_bulk(g)])
+    psi = V.InfiniteMPS.random(1, 2, 16, seed=1)
+    print("svdcut infinite D=16 -> 6", flush=True)
+    psi, envs, _, _ = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=1e-10, maxiter=60, verbosity=0))
+    e16 = V.energy_per_site(psi, H, envs)
+    cut = Cb.svdcut_infinite(psi, Cb.SvdCut(trunc_dim=6))
+    assert tuple(cut.AL[0].shape) == (6, 2, 6)
+    e6 = V.energy_per_site(cut, H, V.InfiniteEnvironments(cut, H, tol=1e-12))
+    assert e6 >= B.tfim_infinite_energy(g) - 1e-12
+    assert abs(e6 - e16) < 1e-6
