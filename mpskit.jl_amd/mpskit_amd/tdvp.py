@@ -3,6 +3,8 @@
 Mirrors (paths relative to the MPSKit.jl tree):
   TDVP  (one site)   src/algorithms/timestep/tdvp.jl:147-210 (_timestep_finite!)
   TDVP2 (two site)   src/algorithms/timestep/tdvp.jl:265-311 (_timestep2_finite!)
+  TDVP, InfiniteMPS  src/algorithms/timestep/tdvp.jl:72-133 (_timestep_infinite: every AC
+                     and C evolved, regauge, right orthogonalisation, new environments)
   integrate          src/algorithms/timestep/integrators.jl:21-31 with
                      imaginary_evolution = true: y = exponentiate(H, -dt, y0)
 
@@ -214,3 +216,25 @@ def time_evolve(psi: FiniteMPS, H, dt, nsteps, alg, envs=None, normalize=False, 
         if callback is not None:
             callback(k, psi, envs)
     return psi, envs, history
+
+
+def timestep_infinite(psi, H, dt, alg: TDVP, envs=None):
+    """``timestep(psi::InfiniteMPS, H, t, dt, TDVP(); imaginary_evolution = true)``
+    (tdvp.jl:91-133, leftorthflag = true): AC[i] <- exp(-dt H_AC) AC[i] and
+    C[i] <- exp(-dt H_C) C[i] for every site with the current environments, AL[i]
+    = regauge(AC[i], C[i]), then the right-orthonormal form and new environments.
+    ``psi`` is a vumps.InfiniteMPS and ``H`` a vumps.InfiniteHamiltonian.
+    Returns (psi, envs, stats)."""
+    from .vumps import InfiniteEnvironments, InfiniteMPS, regauge, uniform_rightorth
+    envs = envs or InfiniteEnvironments(psi, H, tol=1e-12)
+    stats = {"numops": 0, "discarded": 0.0}
+    new_AL = []
+    for s in range(psi.N):
+        AC = _evolve(JordanMPO_AC_Hamiltonian(envs.GLs[s], H.W[s], envs.GRs[s]), -dt, psi.AC[s], alg, stats)
+        C = _evolve(MPO_C_Hamiltonian(envs.GLs[(s + 1) % psi.N], envs.GRs[s]), -dt, psi.C[s], alg, stats)
+        new_AL.append(regauge(T.as_colmajor(AC), T.as_colmajor(C)))
+    AR, C = uniform_rightorth(new_AL, psi.C[-1], tol=1e-14)
+    psi = InfiniteMPS(new_AL, AR, C)
+    envs.recalculate(psi, H, tol=1e-12)
+    return psi, envs, stats
+

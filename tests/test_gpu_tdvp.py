@@ -93,3 +93,18 @@ def test_tdvp_imaginary_time_keeps_the_norm_decay(mk):
     n1 = float(torch.linalg.vector_norm(psi.AC))
     E0 = B.tfim_open_energy(L, g)
     assert abs(np.log(n1 / n0) + 0.05 * E0) < 1e-8
+
+
+def test_infinite_tdvp_imaginary_time_reaches_the_ground_state(mk):
+    """Imaginary-time TDVP on an InfiniteMPS: a VUMPS fixed point is a fixed point
+    of the step, so long runs reach the infinite-TFIM closed form (Pfeuty)."""
+    m, Td = mk
+    from mpskit_amd import vumps as V
+    g = 0.5
+    H = V.InfiniteHamiltonian([m.models.tfim_bulk(g)])
+    psi = V.InfiniteMPS.random(1, 2, 12, seed=3)
+    print("infinite tdvp tfim D=12", flush=True)
+    envs = None
+    for _ in range(40):
+        psi, envs, _ = Td.timestep_infinite(psi, H, 0.5, Td.TDVP(), envs)
+    assert abs(V.energy_per_site(psi, H, envs) - B.tfim_infinite_energy(g)) < 1e-9
