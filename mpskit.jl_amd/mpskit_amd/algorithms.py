@@ -21,7 +21,7 @@ import torch
 from . import tensors as T
 from .derivatives import JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, prepare_operator
 from .environments import FiniteEnvironments
-from .states import FiniteMPS, lq_right, qr_left
+from .states import FiniteMPS, lq_right, pos_qr, qr_left
 
 
 @dataclass
@@ -122,8 +122,25 @@ def truncated_split(M, direction, trunc_dim, trunc_err, method="auto"):
     working precision) and the center as the exact product with it, so no
     singular value is ever divided by; "auto" uses eigh from 512 rows on, where
     the GPU SVD is tens of times slower than the symmetric eigensolver.
+
+    "auto" also recognises splits that discard nothing (trunc_err = 0 and
+    trunc_dim >= min(m, n)): every singular direction is kept, so any isometry
+    with the right span gives the same state (the basis inside the bond is a
+    gauge choice).  If the isometry's side is the smaller one, the isometry is
+    square and the identity serves; otherwise it is the Q of a positive QR (LQ
+    for "left") of M.  S is None for these splits and eps = 0.
     """
     m, n = M.shape
+    if method == "auto" and trunc_err <= 0.0 and trunc_dim >= min(m, n):
+        iso = m if direction == "right" else n
+        if iso <= min(m, n):
+            I = torch.eye(iso, dtype=M.dtype, device=M.device)
+            return (I, M, None, 0.0) if direction == "right" else (M, I, None, 0.0)
+        if direction == "right":
+            Q, R = pos_qr(M)
+            return Q, R, None, 0.0
+        Qt, Rt = pos_qr(M.t())
+        return Rt.t(), Qt.t(), None, 0.0
     if method == "auto":
         method = "eigh" if min(m, n) >= 512 else "svd"
     if method == "svd":
@@ -260,10 +277,15 @@ def local_update(pos, direction, psi, H, alg: DMRG2, envs, eps_global, eps_trunc
     ac2 = psi.AC2(pos)
     y = Heff(ac2)
     y = y / torch.linalg.vector_norm(y)
-    # the center of a sweep is numerically rank deficient (Schmidt values below
-    # roundoff), where Cholesky QR cannot succeed: Householder directly
-    ALp = psi.AL[pos] if psi.center == pos + 1 else qr_left(psi.AC, householder=True)[0]
-    eps_local = float(torch.linalg.vector_norm(project_complement(y, ALp)))
+    Dl, d1, Dr = psi.AC.shape
+    if psi.center == pos and Dl * d1 <= Dr:
+        # the left isometry of the center is square: its complement is empty
+        eps_local = 0.0
+    else:
+        # the center of a sweep is numerically rank deficient (Schmidt values
+        # below roundoff), where Cholesky QR cannot succeed: Householder directly
+        ALp = psi.AL[pos] if psi.center == pos + 1 else qr_left(psi.AC, householder=True)[0]
+        eps_local = float(torch.linalg.vector_norm(project_complement(y, ALp)))
     tol, krylovdim, maxiter = alg.alg_eigsolve.adapt_solver(decay_rate=decay_rate, g_local=eps_local,
                                                             g_global=eps_global, eps_trunc=eps_trunc)
     krylovdim = max(2, min(krylovdim, ac2.numel()))
@@ -307,9 +329,12 @@ def local_update1(pos, direction, psi, H, alg: DMRG, envs, eps_global, decay_rat
                             precisions=precs)
     y = Heff(psi.AC)
     y = y / torch.linalg.vector_norm(y)
-    # calc_galerkin(site, ...) (toolbox.jl:63-70); the center of a converging
-    # sweep is numerically rank deficient, so Householder QR directly
-    eps_local = float(torch.linalg.vector_norm(project_complement(y, qr_left(psi.AC, householder=True)[0])))
+    # calc_galerkin(site, ...) (toolbox.jl:63-70): zero when the left isometry of
+    # the center is square; else Householder QR (the center of a converging
+    # sweep is numerically rank deficient)
+    Dl, d1, Dr = psi.AC.shape
+    eps_local = 0.0 if Dl * d1 <= Dr else float(
+        torch.linalg.vector_norm(project_complement(y, qr_left(psi.AC, householder=True)[0])))
     tol, krylovdim, maxiter = alg.alg_eigsolve.adapt_solver(decay_rate=decay_rate, g_local=eps_local,
                                                             g_global=eps_global, eps_trunc=0.0)
     krylovdim = max(2, min(krylovdim, psi.AC.numel()))

@@ -121,3 +121,24 @@ def test_pos_qr_rank_deficient_and_wide_fall_back():
     W = torch.from_numpy(rng.standard_normal((5, 9)))
     Q, R = pos_qr(W)
     assert Q.shape == (5, 5) and float(torch.linalg.matrix_norm(Q @ R - W)) < 1e-13
+
+
+@pytest.mark.parametrize("direction", ["right", "left"])
+@pytest.mark.parametrize("shape", [(24, 40), (40, 24), (32, 32)])
+def test_truncated_split_without_truncation_is_exact(direction, shape):
+    """Splits that keep every singular direction (trunc_err = 0, trunc_dim >=
+    min(m, n)) return an exact factorisation M = A B with the isometry on the
+    requested side, also for rank-deficient M."""
+    import torch
+    from mpskit_amd.algorithms import truncated_split
+    g = torch.Generator().manual_seed(3)
+    m, n = shape
+    for rank in (min(m, n), min(m, n) // 2):
+        M = torch.randn(m, rank, generator=g, dtype=torch.float64) @ torch.randn(rank, n, generator=g,
+                                                                                 dtype=torch.float64)
+        A, B, S, eps = truncated_split(M, direction, 64, 0.0)
+        assert eps == 0.0 and A.shape[1] == min(m, n) == B.shape[0]
+        assert float(torch.linalg.matrix_norm(A @ B - M)) < 1e-12 * float(torch.linalg.matrix_norm(M))
+        iso = A if direction == "right" else B.t()
+        k = iso.shape[1]
+        assert float(torch.linalg.matrix_norm(iso.t() @ iso - torch.eye(k, dtype=torch.float64))) < 1e-12
