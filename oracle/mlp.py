@@ -3,7 +3,7 @@
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
 module, as the checker of the HIP kernels; the product path never imports it.
 
-The network has no counterpart in the reference: it is the north star's
+The network has no counterpart in the reference: it is this library's
 learned Davidson preconditioner.  Layout of the flat parameter vector:
 [W0 b0 W1 b1 W2 b2], W_l row-major (out x in), layers 512 -> 256 -> 256 -> 256,
 tanh after the first two.  The network output y refines the diagonal

@@ -1,7 +1,7 @@
 """GPU parity of the HIP derivative / transfer kernels against the float64 oracle.
 
 Tolerances (relative Frobenius error of y):
-  fp64   <= 1e-12   (the reference arithmetic; north-star bound)
+  fp64   <= 1e-12   (the reference arithmetic)
   fp32   <= 1e-5    (FP32 MFMA products, fp32 accumulation)
   bf16x3 <= 3e-4    (hi*hi + hi*lo + lo*hi bf16 products, fp32 accumulation)
 """

@@ -1,7 +1,7 @@
 """DMRG2 on the HIP path: ground-state energies against closed forms and exact
 diagonalisation; mixed-precision sweeps against the float64 sweep.
 
-Tolerances: energy per site 1e-10 (north-star bound) where the MPS is exact or
+Tolerances: energy per site 1e-10 where the MPS is exact or
 converged; the L=20 Heisenberg energy of the reference's own test
 (test/algorithms/groundstate.jl:172-174) at its atol 1e-4.
 """
