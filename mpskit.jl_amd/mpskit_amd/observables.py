@@ -1,0 +1,108 @@
+"""Local observables, correlators and entanglement of finite and infinite MPS.
+
+Mirrors (paths relative to the MPSKit.jl tree):
+  expectation_value(psi, i => O)    src/algorithms/expval.jl:53-98 (one site: <AC|O|AC>;
+                                    two neighbouring sites: <AC AR|O|AC AR>; / norm^2)
+  correlator(psi, O1, O2, i, js)    src/algorithms/correlators.jl:11-38 (O1 on the center
+                                    tensor, then right-orthonormal transfers to each j)
+  entanglement_spectrum, entropy    src/algorithms/toolbox.jl:1-30, 80-101 (singular values
+                                    of the bond matrix C; -sum s^2 log s^2 over s >= eps)
+
+Sites are 0-based here.  A cut ``k`` of a finite chain lies between sites k-1 and
+k (k = 0 .. L, the number of sites left of it), which is the reference's
+``site = k`` in its 1-based numbering.  For an InfiniteMPS, cut k lies left of
+site k of the unit cell (k = 0: the bond between cells, the reference's default).
+Operators are real matrices O[b, h] = <b|O|h> (two-site O[b1, b2, h1, h2]).
+The transfers of ``correlator`` are the HIP library's MPS transfer; the small
+contractions at the ends and the SVD of C are torch float64 calls.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import tensors as T
+from .states import FiniteMPS, lq_right
+from .transfer import transfer_left
+
+
+def _op(O, ref):
+    return torch.as_tensor(np.asarray(O, dtype=np.float64), device=ref.device)
+
+
+def _finite(psi):
+    return isinstance(psi, FiniteMPS)
+
+
+def _center(psi, site):
+    """(AC at ``site``, AR at ``site + 1`` or None) in the mixed gauge."""
+    if _finite(psi):
+        psi.move_center(site)
+        return psi.AC, (psi.AR[site + 1] if site + 1 < psi.L else None)
+    N = psi.N
+    return psi.AC[site % N], psi.AR[(site + 1) % N]
+
+
+def _ar(psi, j):
+    return psi.AR[j] if _finite(psi) else psi.AR[j % psi.N]
+
+
+def expectation_value_local(psi, site, O):
+    """<psi| O |psi> / <psi|psi> for a one-site operator at ``site`` or a two-site
+    operator (O of shape (d, d, d, d)) on ``site``, ``site + 1``."""
+    AC, AR = _center(psi, site)
+    O = _op(O, AC)
+    nrm2 = float(torch.dot(AC.reshape(-1), AC.reshape(-1)))
+    if O.dim() == 2:
+        return float(torch.einsum("lbr,bh,lhr->", AC, O, AC)) / nrm2
+    if AR is None:
+        raise ValueError("expectation_value_local: two-site operator on the last site")
+    X = torch.einsum("lbm,mcr->lbcr", AC, AR)
+    return float(torch.einsum("lbcr,bchk,lhkr->", X, O, X)) / nrm2
+
+
+def correlator(psi, O1, O2, i, js):
+    """<O1_i O2_j> / <psi|psi> for every j in ``js`` (all > i), as the reference's
+    correlator: O1 on the center tensor at i, right-orthonormal transfers up to j."""
+    js = list(js)
+    if not js or min(js) <= i:
+        raise ValueError("correlator: every j must be larger than i")
+    AC, _ = _center(psi, i)
+    O1, O2 = _op(O1, AC), _op(O2, AC)
+    nrm2 = float(torch.dot(AC.reshape(-1), AC.reshape(-1)))
+    # V[r_bra, r_ket] = sum AC[l, b, r_bra] O1[b, h] AC[l, h, r_ket]
+    V = T.as_colmajor(torch.einsum("lbr,bh,lhs->rs", AC, O1, AC))
+    out, ctr = [], i + 1
+    for j in sorted(js):
+        while ctr < j:
+            A = _ar(psi, ctr)
+            V = transfer_left(V, A, A)
+            ctr += 1
+        A = _ar(psi, j)
+        out.append(float(torch.einsum("rs,rbt,bh,sht->", V, A, O2, A)) / nrm2)
+    order = {j: k for k, j in enumerate(sorted(js))}
+    return [out[order[j]] for j in js]
+
+
+def entanglement_spectrum(psi, cut=0):
+    """Singular values of the bond matrix C at ``cut`` (descending)."""
+    if _finite(psi):
+        L = psi.L
+        if not 0 <= cut <= L:
+            raise ValueError(f"entanglement_spectrum: cut {cut} outside 0..{L}")
+        if cut == 0 or cut == L:
+            psi.move_center(0 if cut == 0 else L - 1)
+            return torch.linalg.vector_norm(psi.AC).reshape(1)
+        psi.move_center(cut)
+        C, _ = lq_right(psi.AC, householder=True)
+        return torch.linalg.svdvals(C)
+    return torch.linalg.svdvals(psi.C[(cut - 1) % psi.N])
+
+
+def entropy(psi, cut=0):
+    """Von Neumann entropy -sum s^2 log s^2 of the spectrum at ``cut`` (values
+    below machine epsilon contribute zero)."""
+    s = entanglement_spectrum(psi, cut)
+    s = s[s >= torch.finfo(torch.float64).eps]
+    s2 = s * s
+    return float(-(s2 * torch.log(s2)).sum())

@@ -232,3 +232,12 @@ def ising2d_torus_partition(beta, L):
     e = (s * np.roll(s, 1, axis=1)).sum(axis=(1, 2)) + (s * np.roll(s, 1, axis=2)).sum(axis=(1, 2))
     return float(np.exp(beta * e).sum())
 
+
+
+def tfim_infinite_sx(g):
+    """<sigma^x> per site of the infinite TFIM ground state, -d e0 / dg
+    (Hellmann-Feynman on Pfeuty's e0): (1/pi) int_0^pi (g - cos k) / sqrt(1 + g^2 - 2 g cos k) dk."""
+    from scipy.integrate import quad
+    val, _ = quad(lambda k: (g - np.cos(k)) / np.sqrt(1.0 + g * g - 2.0 * g * np.cos(k)), 0.0, np.pi,
+                  epsabs=1e-14, epsrel=1e-14)
+    return val / np.pi

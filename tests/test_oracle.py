@@ -222,3 +222,14 @@ def test_onsager_closed_forms_agree_and_pin_the_critical_value():
     assert abs(crit - (np.log(np.sqrt(2)) + 2 * B.CATALAN / np.pi)) < 1e-12
     assert abs(np.exp(crit) - 2.5337) < 1e-3
     assert abs(B.ising2d_log_partition(1e-9) - np.log(2)) < 1e-12
+
+
+def test_tfim_infinite_sx_is_the_energy_derivative():
+    """<sigma^x> = -d e0 / dg: the closed form against a central difference of
+    Pfeuty's energy, and the limits g -> infinity (1) and g = 0 (0)."""
+    for g in (0.3, 0.8, 1.5):
+        h = 1e-4
+        fd = -(B.tfim_infinite_energy(g + h) - B.tfim_infinite_energy(g - h)) / (2 * h)
+        assert abs(B.tfim_infinite_sx(g) - fd) < 1e-8
+    assert abs(B.tfim_infinite_sx(1e4) - 1.0) < 1e-7
+    assert abs(B.tfim_infinite_sx(0.0)) < 1e-12
