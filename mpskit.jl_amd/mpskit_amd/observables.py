@@ -106,3 +106,31 @@ def entropy(psi, cut=0):
     s = s[s >= torch.finfo(torch.float64).eps]
     s2 = s * s
     return float(-(s2 * torch.log(s2)).sum())
+
+
+def _mpo_product(W1, W2):
+    """(W1 W2)[(g1 g2), b, h, (k1 k2)] = sum_c W1[g1, b, c, k1] W2[g2, c, h, k2]
+    (H * H of the reference's MPO multiplication); the pair index is g1 + w1 g2."""
+    wl1, d, _, wr1 = W1.shape
+    wl2, _, _, wr2 = W2.shape
+    P = np.einsum("abck,gchl->agbhkl", W1, W2)
+    return np.asfortranarray(P.transpose(1, 0, 2, 3, 5, 4).reshape(wl1 * wl2, d, d, wr1 * wr2, order="F"))
+
+
+def variance(psi: FiniteMPS, H):
+    """<H^2> - <H>^2 for a finite MPS (variance(state::FiniteMPS, H), toolbox.jl:120-123):
+    <psi|H H|psi> through the product MPO with the HIP MPO transfers, <H> as energy."""
+    from .algorithms import expectation_value
+    from .environments import FiniteEnvironments
+    L = psi.L
+    psi.move_center(L - 1)
+    G = T.zeros((1, 1, 1))
+    G.fill_(1.0)
+    for j in range(L):
+        W = np.asarray(H.host[j])
+        A = psi.AL[j] if j < L - 1 else psi.AC
+        G = transfer_left(G, T.from_numpy(_mpo_product(W, W)), A, A)
+    nrm2 = float(torch.dot(psi.AC.reshape(-1), psi.AC.reshape(-1)))
+    h2 = float(G.reshape(-1)[0]) / nrm2
+    e = expectation_value(psi, H, FiniteEnvironments(psi, H))
+    return h2 - e * e

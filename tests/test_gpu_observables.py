@@ -87,3 +87,17 @@ def test_infinite_tfim_magnetisation_closed_form(mk_mod):
     s0, s1 = Ob.entanglement_spectrum(psi, 0), Ob.entanglement_spectrum(psi, 1)
     assert abs(float((s0 * s0).sum()) - 1.0) < 1e-12
     assert abs(Ob.entropy(psi, 0) - Ob.entropy(psi, 1)) < 1e-8
+
+
+def test_variance_finite(mk_mod, ground):
+    """variance = <H^2> - <H>^2: ~0 for the exact ground state, the dense value
+    for a random state (product MPO through the HIP MPO transfers)."""
+    m, Ob = mk_mod
+    psi, v, L = ground
+    H = m.FiniteMPOHamiltonian(m.models.tfim(L, 1.5))
+    assert abs(Ob.variance(psi, H)) < 1e-9
+    rnd = m.FiniteMPS.random(L, 2, 8, seed=9)
+    w = rnd.to_dense().cpu().numpy()
+    Hd = B.dense_tfim(L, 1.5).toarray()
+    ref = (w @ Hd @ Hd @ w) / (w @ w) - ((w @ Hd @ w) / (w @ w)) ** 2
+    assert abs(Ob.variance(rnd, H) - ref) < 1e-9 * max(1.0, abs(ref))
