@@ -134,3 +134,15 @@ def variance(psi: FiniteMPS, H):
     h2 = float(G.reshape(-1)[0]) / nrm2
     e = expectation_value(psi, H, FiniteEnvironments(psi, H))
     return h2 - e * e
+
+
+def dot(psi1: FiniteMPS, psi2: FiniteMPS):
+    """<psi1|psi2> of two finite MPS on the same chain (dot(psi1, psi2) of the
+    reference, states/finitemps.jl): the left MPS transfer through all sites."""
+    if psi1.L != psi2.L:
+        raise ValueError("dot: chains of different length")
+    V = T.zeros((1, 1))
+    V.fill_(1.0)
+    for j in range(psi1.L):
+        V = transfer_left(V, psi2.tensor_at(j), psi1.tensor_at(j))
+    return float(V.reshape(-1)[0])

@@ -101,3 +101,14 @@ def test_variance_finite(mk_mod, ground):
     Hd = B.dense_tfim(L, 1.5).toarray()
     ref = (w @ Hd @ Hd @ w) / (w @ w) - ((w @ Hd @ w) / (w @ w)) ** 2
     assert abs(Ob.variance(rnd, H) - ref) < 1e-9 * max(1.0, abs(ref))
+
+
+def test_dot_of_two_states(mk_mod, ground):
+    m, Ob = mk_mod
+    psi, v, L = ground
+    other = m.FiniteMPS.random(L, 2, 8, seed=10)
+    other.move_center(3)
+    w = other.to_dense().cpu().numpy()
+    vg = psi.to_dense().cpu().numpy()
+    assert abs(Ob.dot(psi, other) - float(vg @ w)) < 1e-12
+    assert abs(Ob.dot(other, other) - float(w @ w)) < 1e-12
