@@ -146,3 +146,66 @@ def dot(psi1: FiniteMPS, psi2: FiniteMPS):
     for j in range(psi1.L):
         V = transfer_left(V, psi2.tensor_at(j), psi1.tensor_at(j))
     return float(V.reshape(-1)[0])
+
+
+def transfer_spectrum(psi, howmany=20, krylovdim=None, seed=0):
+    """Leading (largest modulus) eigenvalues of the left transfer matrix of the
+    unit cell of an InfiniteMPS (transfer_spectrum, transfer_spectrum.jl:27-60,
+    one dense sector): one Arnoldi space of ``krylovdim`` (default max(30,
+    2 howmany + 10)) vectors over the HIP MPS transfer, two Gram-Schmidt passes
+    per step.  Returns (eigenvalues as a complex numpy array in descending
+    modulus, their Ritz residual estimates)."""
+    from .vumps import _flat, _unflat
+    D = psi.AL[0].shape[0]
+    n = D * D
+    k = min(krylovdim or max(30, 2 * howmany + 10), n)
+    rng = np.random.default_rng(seed)
+    v = T.from_numpy(rng.standard_normal((D, D)))
+    Vb = torch.empty((k + 1, n), dtype=torch.float64, device=v.device)
+    x = _flat(v)
+    Vb[0] = x / torch.linalg.vector_norm(x)
+    H = np.zeros((k + 1, k))
+    m = k
+    for j in range(k):
+        y = _unflat(Vb[j], (D, D))
+        for s in range(psi.N):
+            y = transfer_left(T.as_colmajor(y), psi.AL[s], psi.AL[s])
+        w = _flat(y)
+        Vj = Vb[: j + 1]
+        h1 = Vj @ w
+        w = w - Vj.t() @ h1
+        h2 = Vj @ w
+        w = w - Vj.t() @ h2
+        hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+        H[: j + 1, j] = hcol[:-1]
+        H[j + 1, j] = hcol[-1]
+        if hcol[-1] <= 1e-14:
+            m = j + 1
+            break
+        Vb[j + 1] = w / hcol[-1]
+    ev, U = np.linalg.eig(H[:m, :m])
+    order = np.argsort(-np.abs(ev))[: min(howmany, m)]
+    res = np.abs(H[m, m - 1] * U[m - 1, order]) if m < H.shape[0] else np.zeros(len(order))
+    return ev[order], res
+
+
+def marek_gap(spectrum, tol_angle=0.1):
+    """(epsilon, delta, theta) of a transfer spectrum (marek_gap, transfer_spectrum.jl:136-160):
+    drop the eigenvalues of modulus 1, take the angle of the largest remaining one,
+    epsilon = -log|lambda| of the first, delta the gap to the next at that angle."""
+    spec = np.asarray(spectrum)
+    spec = spec[np.abs(spec) < 1 - 1e-12]
+    spec = spec[np.argsort(-np.abs(spec))]
+    angles = np.angle(spec)
+    theta = angles[0]
+    # same angle: within tol_angle in units of pi, as approx_angles rationalises angle / pi
+    at = spec[np.abs(np.angle(np.exp(1j * (angles - theta)))) / np.pi < tol_angle]
+    lambdas = -np.log(np.abs(at))
+    delta = lambdas[1] - lambdas[0] if len(lambdas) > 1 else np.inf
+    return float(lambdas[0]), float(delta), float(theta)
+
+
+def correlation_length(psi, **kw):
+    """1 / epsilon of the transfer spectrum (correlation_length, transfer_spectrum.jl:170-180)."""
+    ev, _ = transfer_spectrum(psi, **kw)
+    return 1.0 / marek_gap(ev)[0]

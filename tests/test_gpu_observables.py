@@ -112,3 +112,25 @@ def test_dot_of_two_states(mk_mod, ground):
     vg = psi.to_dense().cpu().numpy()
     assert abs(Ob.dot(psi, other) - float(vg @ w)) < 1e-12
     assert abs(Ob.dot(other, other) - float(w @ w)) < 1e-12
+
+
+def test_transfer_spectrum_and_correlation_length(mk_mod):
+    """Infinite TFIM paramagnet (g = 1.5): the transfer matrix of the normalised
+    MPS has the eigenvalue 1; its next eigenvalue sets the asymptotic decay of
+    the connected <sigma^z sigma^z> computed with ``correlator``, and the
+    correlation length approaches the exact 1 / ln g from below as D grows."""
+    m, Ob = mk_mod
+    from mpskit_amd import vumps as V
+    g = 1.5
+    H = V.InfiniteHamiltonian([m.models.tfim_bulk(g)])
+    psi = V.InfiniteMPS.random(1, 2, 24, seed=5)
+    print("transfer spectrum tfim g=1.5 D=24", flush=True)
+    psi, envs, _, _ = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=1e-11, maxiter=80, verbosity=0))
+    ev, res = Ob.transfer_spectrum(psi, howmany=6)
+    assert abs(ev[0] - 1.0) < 1e-10 and res[0] < 1e-8
+    xi = Ob.correlation_length(psi, howmany=6)
+    xi_exact = 1.0 / np.log(g)
+    assert 0.8 * xi_exact < xi < xi_exact * (1 + 1e-6)
+    # <sigma^z> = 0 in the paramagnet: the correlator is connected; its decay rate
+    c = Ob.correlator(psi, B.PZ, B.PZ, 0, [30, 31])
+    assert abs(c[1] / c[0] / np.exp(-1.0 / xi) - 1.0) < 1e-2
