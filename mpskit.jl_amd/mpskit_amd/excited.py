@@ -191,7 +191,8 @@ def find_excited(psi: FiniteMPS, H, states, alg: FiniteExcited):
 def excitations(H, alg: FiniteExcited, states, init: FiniteMPS | None = None, num: int = 1):
     """``excitations(H, FiniteExcited(), states; init, num)`` (dmrgexcitation.jl:37-70):
     the ``num`` next states above ``states``, each penalising all found before it.
-    Returns (energies, excited states)."""
+    The gauge center of the given states moves during the sweeps (the states
+    themselves do not change).  Returns (energies, excited states)."""
     states = list(states) if isinstance(states, (list, tuple)) else [states]
     energies, found = [], []
     for _ in range(num):
