@@ -50,15 +50,15 @@ class InfiniteMPO:
         self.N = len(Os)
 
 
-def _left_cell(v, psi, O):
+def _left_cell(v, psi, O, ket):
     for s in range(psi.N):
-        v = transfer_left(v, O.O[s], psi.AL[s], psi.AL[s])
+        v = transfer_left(v, O.O[s], ket.AL[s], psi.AL[s])
     return v
 
 
-def _right_cell(v, psi, O):
+def _right_cell(v, psi, O, ket):
     for s in range(psi.N - 1, -1, -1):
-        v = transfer_right(v, O.O[s], psi.AR[s], psi.AR[s])
+        v = transfer_right(v, O.O[s], ket.AR[s], psi.AR[s])
     return v
 
 
@@ -67,11 +67,14 @@ def _start(shape, seed):
 
 
 class BoundaryEnvironments:
-    """GLs[i] (D x w x D): left environment of site i; GRs[i]: right environment
-    of site i.  ``lambdas`` holds the cell eigenvalues of the last solve."""
+    """GLs[i] (D_bra x w x D_ket): left environment of site i; GRs[i] (D_ket x w x
+    D_bra): right environment of site i.  The bra is the state passed to
+    ``recalculate``; the ket is ``ket`` (default: the bra itself, as for
+    leading_boundary; another state for approximate).  ``lambdas`` holds the cell
+    eigenvalues of the last solve."""
 
-    def __init__(self, psi, O, tol=1e-12, krylovdim=20):
-        self.tol, self.krylovdim = tol, krylovdim
+    def __init__(self, psi, O, tol=1e-12, krylovdim=20, ket=None):
+        self.tol, self.krylovdim, self.ket = tol, krylovdim, ket
         self.GLs = self.GRs = None
         self.lambdas = (None, None)
         self.numops = 0
@@ -79,36 +82,34 @@ class BoundaryEnvironments:
 
     def recalculate(self, psi, O, tol=None):
         tol = self.tol if tol is None else tol
+        ket = self.ket if self.ket is not None else psi
         N = psi.N
-        gl0 = self.GLs[0] if self.GLs is not None else _start(
-            (psi.AL[0].shape[0], O.host[0].shape[0], psi.AL[0].shape[0]), 1)
-        gr0 = self.GRs[N - 1] if self.GRs is not None else _start(
-            (psi.AR[N - 1].shape[2], O.host[N - 1].shape[3], psi.AR[N - 1].shape[2]), 2)
-        if tuple(gl0.shape) != (psi.AL[0].shape[0], O.host[0].shape[0], psi.AL[0].shape[0]):
-            gl0 = _start((psi.AL[0].shape[0], O.host[0].shape[0], psi.AL[0].shape[0]), 1)
-        if tuple(gr0.shape) != (psi.AR[N - 1].shape[2], O.host[N - 1].shape[3], psi.AR[N - 1].shape[2]):
-            gr0 = _start((psi.AR[N - 1].shape[2], O.host[N - 1].shape[3], psi.AR[N - 1].shape[2]), 2)
+        shape_l = (psi.AL[0].shape[0], O.host[0].shape[0], ket.AL[0].shape[0])
+        shape_r = (ket.AR[N - 1].shape[2], O.host[N - 1].shape[3], psi.AR[N - 1].shape[2])
+        gl0 = self.GLs[0] if self.GLs is not None and tuple(self.GLs[0].shape) == shape_l else _start(shape_l, 1)
+        gr0 = (self.GRs[N - 1] if self.GRs is not None and tuple(self.GRs[N - 1].shape) == shape_r
+               else _start(shape_r, 2))
         # compute_leftenvs! / compute_rightenvs! (infinite_envs.jl:125-156)
         il, ir = {}, {}
-        lam_l, gl = arnoldi_dominant(lambda v: _left_cell(v, psi, O), gl0, tol=tol, krylovdim=self.krylovdim,
+        lam_l, gl = arnoldi_dominant(lambda v: _left_cell(v, psi, O, ket), gl0, tol=tol, krylovdim=self.krylovdim,
                                      relative=True, info=il)
-        lam_r, gr = arnoldi_dominant(lambda v: _right_cell(v, psi, O), gr0, tol=tol, krylovdim=self.krylovdim,
+        lam_r, gr = arnoldi_dominant(lambda v: _right_cell(v, psi, O, ket), gr0, tol=tol, krylovdim=self.krylovdim,
                                      relative=True, info=ir)
         self.numops += il["numops"] + ir["numops"]
         GL = [None] * N
         GR = [None] * N
         GL[0] = T.as_colmajor(gl)
         for s in range(1, N):
-            GL[s] = transfer_left(GL[s - 1], O.O[s - 1], psi.AL[s - 1], psi.AL[s - 1])
+            GL[s] = transfer_left(GL[s - 1], O.O[s - 1], ket.AL[s - 1], psi.AL[s - 1])
         GR[N - 1] = T.as_colmajor(gr)
         for s in range(N - 2, -1, -1):
-            GR[s] = transfer_right(GR[s + 1], O.O[s + 1], psi.AR[s + 1], psi.AR[s + 1])
-        # normalize! (infinite_envs.jl:163-175): |GR| = 1, <C|dC(C)> = 1 through GL[i+1]
+            GR[s] = transfer_right(GR[s + 1], O.O[s + 1], ket.AR[s + 1], psi.AR[s + 1])
+        # normalize! (infinite_envs.jl:163-175): |GR| = 1, <C_bra|dC(C_ket)> = 1 through GL[i+1]
         for s in range(N):
             GR[s] = GR[s] / _norm(GR[s])
             nxt = (s + 1) % N
             Hc = prepare_operator(MPO_C_Hamiltonian(GL[nxt], GR[s]))
-            lam = float(_dot(psi.C[s], Hc(psi.C[s])))
+            lam = float(_dot(psi.C[s], Hc(ket.C[s])))
             Hc.free()
             GL[nxt] = GL[nxt] / lam
         self.GLs, self.GRs = GL, GR
@@ -128,11 +129,13 @@ def expectation_value_mpo(psi, O, envs):
 
 
 def calc_galerkin_mpo(psi, O, envs):
-    """max over sites of |(1 - AL AL^T) normalize(dAC(AC))| (toolbox.jl:63-79)."""
+    """max over sites of |(1 - AL AL^T) normalize(dAC(AC_ket))| (toolbox.jl:63-79),
+    AC_ket the ket's center tensor (psi's own for leading_boundary)."""
+    ket = envs.ket if envs.ket is not None else psi
     worst = 0.0
     for s in range(psi.N):
         op = prepare_operator(MPO_AC_Hamiltonian(envs.GLs[s], O.O[s], envs.GRs[s]))
-        y = op(psi.AC[s])
+        y = op(ket.AC[s])
         y = y / _norm(y)
         Xm, Ym = T.fuse(psi.AL[s], 2), T.fuse(y, 2)
         worst = max(worst, float(_norm(Ym - Xm @ (Xm.t() @ Ym))))
@@ -231,3 +234,41 @@ def leading_boundary(psi: InfiniteMPS, O: InfiniteMPO, alg):
         if eps <= alg.tol:
             break
     return psi, envs, eps, history
+
+
+def approximate_infinite(psi: InfiniteMPS, O: InfiniteMPO, phi: InfiniteMPS, alg: VOMPS):
+    """``approximate(psi, (O, phi), VOMPS(...))`` (approximate/vomps.jl:15-110): psi
+    approaches O|phi> (up to normalisation) within its bond dimension.  Each
+    iteration sets AC = dAC(phi.AC) and C = dC(phi.C) with the mixed
+    environments (bra psi, ket phi), regauges and recomputes the environments.
+    Returns (psi, envs, eps, history)."""
+    envs = BoundaryEnvironments(psi, O, tol=1e-10, ket=phi)
+    eps = calc_galerkin_mpo(psi, O, envs)
+    history = []
+    precs = ("fp64",) if alg.precision == "fp64" else ("fp64", alg.precision)
+    for it in range(1, alg.maxiter + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tol = min(max(alg.tol_factor * eps, alg.tol_min), alg.tol_max)
+        new_AL = []
+        for s in range(psi.N):
+            nxt = (s + 1) % psi.N
+            Hac = prepare_operator(MPO_AC_Hamiltonian(envs.GLs[s], O.O[s], envs.GRs[s]), precisions=precs)
+            Hc = prepare_operator(MPO_C_Hamiltonian(envs.GLs[nxt], envs.GRs[s]), precisions=precs)
+            AC = Hac(phi.AC[s], precision=alg.precision)
+            C = Hc(phi.C[s], precision=alg.precision)
+            Hac.free()
+            Hc.free()
+            new_AL.append(regauge(AC, C))
+        AR, C = uniform_rightorth(new_AL, psi.C[-1], tol=max(tol, 1e-14))
+        psi = InfiniteMPS(new_AL, AR, C)
+        envs.recalculate(psi, O, tol=max(tol * 0.1, 1e-14))
+        eps = calc_galerkin_mpo(psi, O, envs)
+        torch.cuda.synchronize()
+        history.append(dict(iter=it, eps=eps, time=time.perf_counter() - t0))
+        if alg.verbosity >= 2:
+            print(f"approximate VOMPS {it:3d}: err = {eps:.3e}", flush=True)
+        if eps <= alg.tol:
+            break
+    return psi, envs, eps, history
+

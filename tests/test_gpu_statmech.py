@@ -68,3 +68,42 @@ def test_vumps_boundary_ising_critical(mk):
     logz, eps, _ = _run(mk, beta, 10, Bd.BoundaryVUMPS(tol=1e-4, maxiter=100, verbosity=1))
     assert abs(np.exp(logz) - 2.5337) < 1e-3
     assert abs(logz - B.ising2d_log_partition(beta)) < 1e-3
+
+
+def test_approximate_infinite_rotation_mpo(mk):
+    """An on-site orthogonal rotation R as a w = 1 MPO maps phi to an MPS of the
+    same bond dimension; approximate must reproduce it: <sigma>_psi equals
+    <R^T sigma R>_phi."""
+    m, Bd, V = mk
+    from mpskit_amd import observables as Ob
+    t = 0.4
+    R = np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]])
+    O = Bd.InfiniteMPO([R[None, :, :, None]])
+    phi = V.InfiniteMPS.random(1, 2, 8, seed=11)
+    psi = V.InfiniteMPS.random(1, 2, 8, seed=12)
+    print("approximate infinite rotation", flush=True)
+    psi, envs, eps, hist = Bd.approximate_infinite(psi, O, phi, Bd.VOMPS(tol=1e-11, maxiter=100, verbosity=1))
+    assert eps <= 1e-10
+    for P in (B.PZ, B.PX):
+        want = Ob.expectation_value_local(phi, 0, R.T @ P @ R)
+        assert abs(Ob.expectation_value_local(psi, 0, P) - want) < 1e-10
+
+
+def test_approximate_infinite_returns_the_boundary_fixed_point(mk):
+    """The Ising transfer MPO applied to its own boundary MPS is that MPS again
+    (up to normalisation): same entanglement spectrum."""
+    m, Bd, V = mk
+    from mpskit_amd import observables as Ob
+    O = Bd.InfiniteMPO([m.models.classical_ising(0.3)])
+    phi = V.InfiniteMPS.random(1, 2, 12, seed=0)
+    print("approximate infinite boundary fixed point", flush=True)
+    phi, _, _, _ = Bd.leading_boundary(phi, O, Bd.BoundaryVUMPS(tol=1e-12, maxiter=80, verbosity=0))
+    psi = V.InfiniteMPS.random(1, 2, 12, seed=13)
+    psi, envs, eps, hist = Bd.approximate_infinite(psi, O, phi, Bd.VOMPS(tol=1e-10, maxiter=200, verbosity=1))
+    assert eps <= 1e-10
+    s_phi, s_psi = Ob.entanglement_spectrum(phi, 0), Ob.entanglement_spectrum(psi, 0)
+    assert float(torch_max_abs(s_phi - s_psi)) < 1e-7
+
+
+def torch_max_abs(t):
+    return t.abs().max()
