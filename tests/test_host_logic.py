@@ -175,3 +175,12 @@ def test_svdcut_keep_rule():
     assert _keep(S, SvdCut(trunc_dim=3)) == 3
     assert _keep(S, SvdCut(trunc_dim=10, trunc_err=1e-5)) == 3     # tail 2-norm of [1e-6, 1e-9] < 1e-5
     assert _keep(S, SvdCut(trunc_dim=10, trunc_err=0.0)) == 5
+
+
+def test_infinite_mpo_rejects_inconsistent_bonds():
+    """InfiniteMPO checks its unit cell before touching the device."""
+    from mpskit_amd.boundary import InfiniteMPO
+    with pytest.raises(ValueError):
+        InfiniteMPO([np.zeros((2, 2, 2, 3))])            # right bond 3 != left bond 2 of the next site
+    with pytest.raises(ValueError):
+        InfiniteMPO([np.zeros((2, 2, 3, 2))])            # physical legs 2 and 3
