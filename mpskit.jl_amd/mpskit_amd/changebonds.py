@@ -7,6 +7,8 @@ Mirrors (paths relative to the MPSKit.jl tree):
                  ``trunc_dim`` singular directions added to the bond
   _expand!       src/algorithms/changebonds/changebonds.jl:59-84: AL -> [AL expansion;
                  0 0], AR -> [AR 0; expansion 0], C -> [C 0; 0 0] (the state is unchanged)
+  finite         src/algorithms/changebonds/optimalexpand.jl:112-140 (changebond! :right per
+                 bond, left to right): AC -> [AC 0] re-gauged, AR -> [AR; added rows]
   SvdCut         src/algorithms/changebonds/svdcut.jl: truncated SVD of every bond
                  matrix C in the mixed-canonical gauge, right to left for a finite
                  MPS; for an infinite MPS AL[i] U, U^T AL[i+1] and a new gauge fix
@@ -24,7 +26,7 @@ import torch
 
 from . import tensors as T
 from .derivatives import JordanMPO_AC2_Hamiltonian, MPO_AC2_Hamiltonian, prepare_operator
-from .states import FiniteMPS, lq_right
+from .states import FiniteMPS, lq_right, qr_left
 from .vumps import InfiniteMPS
 
 
@@ -152,4 +154,56 @@ def svdcut_infinite(psi: InfiniteMPS, alg: SvdCut) -> InfiniteMPS:
 
 def _copy_finite(psi: FiniteMPS) -> FiniteMPS:
     return FiniteMPS([T.as_colmajor(psi.tensor_at(j).clone()) for j in range(psi.L)])
+
+
+def _complement_cols(M):
+    """Orthonormal basis of the complement of range(M) (numerical rank by SVD)."""
+    U, S, _ = torch.linalg.svd(M, full_matrices=True)
+    r = int((S > S[0] * M.shape[0] * torch.finfo(torch.float64).eps).sum()) if S.numel() else 0
+    return U[:, r:]
+
+
+def changebonds_finite(psi: FiniteMPS, H, alg: OptimalExpand, envs):
+    """changebonds!(psi::FiniteMPS, H, OptimalExpand(...), envs) (optimalexpand.jl:112-175):
+    every bond, left to right, grows by at most ``trunc_dim`` directions taken from
+    the two-site update projected onto the complement of the current state; the
+    state is unchanged (the new directions carry zero weight).  ``envs`` are the
+    FiniteEnvironments of (psi, H); the center ends on the last site."""
+    for i in range(psi.L - 1):
+        psi.move_center(i)
+        left, right = psi.AC, psi.AR[i + 1]
+        Dl, d, D = left.shape
+        d2, Dr = right.shape[1], right.shape[2]
+        NL = _complement_cols(T.fuse(left, 2))
+        NR = _complement_cols(T.fuse(right, 1).t()).t()
+        k = min(alg.trunc_dim, NL.shape[1], NR.shape[0])
+        if k > 0:
+            op = prepare_operator(JordanMPO_AC2_Hamiltonian(envs.leftenv(i, psi), H[i], H[i + 1],
+                                                            envs.rightenv(i + 1, psi)))
+            y = op(psi.AC2(i))
+            op.free()
+            Y = T.fuse(T.as_colmajor(y.permute(0, 1, 3, 2)), 2)
+            g2 = NL.t() @ Y @ NR.t()
+            nrm = float(torch.linalg.matrix_norm(g2))
+            if nrm <= torch.finfo(torch.float64).eps ** 0.75:
+                k = 0
+        if k == 0:
+            AL, C = qr_left(left, householder=True)
+            psi.AL[i] = AL
+            nxt = torch.einsum("lm,mpr->lpr", C, right)
+        else:
+            _, _, Vh = torch.linalg.svd(g2 / nrm, full_matrices=False)
+            ar_re = T.unfuse(Vh[:k] @ NR, (k, d2, Dr))
+            ext = T.zeros((Dl, d, D + k))
+            ext[:, :, :D] = left
+            AL, C = qr_left(ext, householder=True)
+            nar = T.zeros((D + k, d2, Dr))
+            nar[:D] = right
+            nar[D:] = ar_re
+            psi.AL[i] = AL
+            nxt = torch.einsum("lm,mpr->lpr", C, nar)
+        psi.AR[i + 1] = None
+        psi.AC = T.as_colmajor(nxt / torch.linalg.vector_norm(nxt))
+        psi.center = i + 1
+    return psi
 

@@ -93,3 +93,29 @@ def test_svdcut_infinite_tfim(mk):
     e6 = V.energy_per_site(cut, H, V.InfiniteEnvironments(cut, H, tol=1e-12))
     assert e6 >= B.tfim_infinite_energy(g) - 1e-12
     assert abs(e6 - e16) < 1e-6
+
+
+def test_optimal_expand_finite_with_single_site_dmrg(mk):
+    """Single-site DMRG keeps its bonds; finite OptimalExpand grows them without
+    changing the state, and alternating the two reaches the full bond dimension
+    and the open-TFIM closed form."""
+    m, Bd, Cb, V = mk
+    L, g = 10, 1.0
+    H = m.FiniteMPOHamiltonian(m.models.tfim(L, g))
+    psi = m.FiniteMPS.random(L, 2, 2, seed=7)
+    envs = m.environments(psi, H)
+    print("finite expansion with DMRG1", flush=True)
+    full = [min(2 ** k, 2 ** (L - k)) for k in range(L + 1)]
+    for _ in range(10):
+        psi, envs, _, hist = m.find_groundstate(psi, H, m.DMRG(tol=1e-12, maxiter=6, verbosity=0), envs)
+        if psi.bond_dims() == full:
+            break
+        before = m.expectation_value(psi, H, envs)
+        dims = psi.bond_dims()
+        psi = Cb.changebonds_finite(psi, H, Cb.OptimalExpand(trunc_dim=8), envs)
+        print("bond dims", psi.bond_dims(), flush=True)
+        assert abs(m.expectation_value(psi, H, envs) - before) < 1e-10
+        assert all(b >= a for a, b in zip(dims, psi.bond_dims())) and psi.bond_dims() != dims
+    psi, envs, _, hist = m.find_groundstate(psi, H, m.DMRG(tol=1e-12, maxiter=10, verbosity=0), envs)
+    assert psi.bond_dims() == full
+    assert abs(hist[-1]["energy"] - B.tfim_open_energy(L, g)) / L < 1e-10
