@@ -31,10 +31,41 @@ from .vumps import InfiniteMPS
 
 
 @dataclass
+class RandExpand:
+    """RandExpand(; trunc = truncrank(trunc_dim)) (randexpand.jl): at most
+    ``trunc_dim`` random directions from the null spaces of AL and AR added to
+    every bond (no Hamiltonian involved)."""
+    trunc_dim: int = 1
+    seed: int = 0
+
+
+@dataclass
 class OptimalExpand:
     """OptimalExpand(; trunc = truncrank(trunc_dim)): at most ``trunc_dim``
     directions are added to every bond."""
     trunc_dim: int = 1
+
+
+def _expand_infinite(psi, ALx, ARx):
+    """_expand! (changebonds.jl:59-84): ALx[i] are columns added to AL[i], ARx[i]
+    rows added to AR[i]; the new directions carry zero weight in C."""
+    N = psi.N
+    AL, AR, C = [None] * N, [None] * N, [None] * N
+    for i in range(N):
+        kl, kr = ALx[i - 1].shape[2], ALx[i].shape[2]     # added to the left / right bond of site i
+        Dl, d, Dr = psi.AL[i].shape
+        a = T.zeros((Dl + kl, d, Dr + kr))
+        a[:Dl, :, :Dr] = psi.AL[i]
+        a[:Dl, :, Dr:] = ALx[i]
+        AL[i] = a
+        b = T.zeros((Dl + kl, d, Dr + kr))
+        b[:Dl, :, :Dr] = psi.AR[i]
+        b[Dl:, :, :Dr] = ARx[i]
+        AR[i] = b
+        c = T.zeros((Dr + kr, Dr + kr))
+        c[:Dr, :Dr] = psi.C[i]
+        C[i] = c
+    return InfiniteMPS(AL, AR, C)
 
 
 def _left_null(AL):
@@ -87,23 +118,7 @@ def changebonds(psi: InfiniteMPS, H, alg: OptimalExpand, envs):
         U, _, Vh = torch.linalg.svd(X, full_matrices=False)
         ALx[i] = T.as_colmajor(T.unfuse(VL @ U[:, :k], (Dl, d, k)))
         ARx[j] = T.as_colmajor(T.unfuse(Vh[:k] @ VR, (k, d2, Dr2)))
-    # _expand! (changebonds.jl:59-84)
-    AL, AR, C = [None] * N, [None] * N, [None] * N
-    for i in range(N):
-        kl, kr = ALx[i - 1].shape[2], ALx[i].shape[2]     # added to the left / right bond of site i
-        Dl, d, Dr = psi.AL[i].shape
-        a = T.zeros((Dl + kl, d, Dr + kr))
-        a[:Dl, :, :Dr] = psi.AL[i]
-        a[:Dl, :, Dr:] = ALx[i]
-        AL[i] = a
-        b = T.zeros((Dl + kl, d, Dr + kr))
-        b[:Dl, :, :Dr] = psi.AR[i]
-        b[Dl:, :, :Dr] = ARx[i]
-        AR[i] = b
-        c = T.zeros((Dr + kr, Dr + kr))
-        c[:Dr, :Dr] = psi.C[i]
-        C[i] = c
-    return InfiniteMPS(AL, AR, C)
+    return _expand_infinite(psi, ALx, ARx)
 
 
 @dataclass
@@ -163,12 +178,15 @@ def _complement_cols(M):
     return U[:, r:]
 
 
-def changebonds_finite(psi: FiniteMPS, H, alg: OptimalExpand, envs):
+def changebonds_finite(psi: FiniteMPS, H, alg, envs):
     """changebonds!(psi::FiniteMPS, H, OptimalExpand(...), envs) (optimalexpand.jl:112-175):
     every bond, left to right, grows by at most ``trunc_dim`` directions taken from
     the two-site update projected onto the complement of the current state; the
     state is unchanged (the new directions carry zero weight).  ``envs`` are the
-    FiniteEnvironments of (psi, H); the center ends on the last site."""
+    FiniteEnvironments of (psi, H); the center ends on the last site.  With
+    ``alg`` a RandExpand the directions come from a random update instead (H and
+    envs unused)."""
+    gen = torch.Generator(device=psi.AC.device).manual_seed(getattr(alg, "seed", 0))
     for i in range(psi.L - 1):
         psi.move_center(i)
         left, right = psi.AC, psi.AR[i + 1]
@@ -177,7 +195,11 @@ def changebonds_finite(psi: FiniteMPS, H, alg: OptimalExpand, envs):
         NL = _complement_cols(T.fuse(left, 2))
         NR = _complement_cols(T.fuse(right, 1).t()).t()
         k = min(alg.trunc_dim, NL.shape[1], NR.shape[0])
-        if k > 0:
+        if k > 0 and isinstance(alg, RandExpand):
+            # randexpand.jl: a randomised two-site update restricted to the complement
+            g2 = torch.randn(NL.shape[1], NR.shape[0], generator=gen, dtype=torch.float64, device=NL.device)
+            nrm = float(torch.linalg.matrix_norm(g2))
+        elif k > 0:
             op = prepare_operator(JordanMPO_AC2_Hamiltonian(envs.leftenv(i, psi), H[i], H[i + 1],
                                                             envs.rightenv(i + 1, psi)))
             y = op(psi.AC2(i))
@@ -206,4 +228,26 @@ def changebonds_finite(psi: FiniteMPS, H, alg: OptimalExpand, envs):
         psi.AC = T.as_colmajor(nxt / torch.linalg.vector_norm(nxt))
         psi.center = i + 1
     return psi
+
+
+def randexpand_infinite(psi: InfiniteMPS, alg: RandExpand) -> InfiniteMPS:
+    """changebonds!(psi::InfiniteMPS, RandExpand(...)) (randexpand.jl:28-44):
+    random isometries into the left null space of AL[i] and the right null space
+    of AR[i+1]."""
+    g = torch.Generator(device=psi.AL[0].device).manual_seed(alg.seed)
+    N = psi.N
+    ALx, ARx = [None] * N, [None] * N
+    for i in range(N):
+        j = (i + 1) % N
+        VL, VR = _left_null(psi.AL[i]), _right_null(psi.AR[j])
+        k = min(alg.trunc_dim, VL.shape[1], VR.shape[0])
+        Dl, d, _ = psi.AL[i].shape
+        d2, Dr2 = psi.AR[j].shape[1], psi.AR[j].shape[2]
+        XL, _ = torch.linalg.qr(torch.randn(VL.shape[1], max(k, 1), generator=g, dtype=torch.float64,
+                                            device=VL.device))
+        XR, _ = torch.linalg.qr(torch.randn(VR.shape[0], max(k, 1), generator=g, dtype=torch.float64,
+                                            device=VR.device))
+        ALx[i] = T.as_colmajor(T.unfuse(VL @ XL[:, :k], (Dl, d, k)))
+        ARx[j] = T.as_colmajor(T.unfuse(XR[:, :k].t() @ VR, (k, d2, Dr2)))
+    return _expand_infinite(psi, ALx, ARx)
 

@@ -119,3 +119,27 @@ def test_optimal_expand_finite_with_single_site_dmrg(mk):
     psi, envs, _, hist = m.find_groundstate(psi, H, m.DMRG(tol=1e-12, maxiter=10, verbosity=0), envs)
     assert psi.bond_dims() == full
     assert abs(hist[-1]["energy"] - B.tfim_open_energy(L, g)) / L < 1e-10
+
+
+def test_randexpand_infinite_and_finite_preserve_the_state(mk):
+    m, Bd, Cb, V = mk
+    g = 0.5
+    H = V.InfiniteHamiltonian([m.models.tfim_bulk(g)] * 2)
+    psi = V.InfiniteMPS.random(2, 2, 6, seed=1)
+    print("randexpand", flush=True)
+    psi, envs, _, _ = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=1e-8, maxiter=60, verbosity=0))
+    e0 = V.energy_per_site(psi, H, envs)
+    big = Cb.randexpand_infinite(psi, Cb.RandExpand(trunc_dim=3, seed=2))
+    assert all(tuple(a.shape) == (9, 2, 9) for a in big.AL)
+    assert abs(V.energy_per_site(big, H, V.InfiniteEnvironments(big, H, tol=1e-12)) - e0) < 1e-9
+    L = 10
+    Hf = m.FiniteMPOHamiltonian(m.models.tfim(L, 1.0))
+    f = m.FiniteMPS.random(L, 2, 4, seed=3)
+    envs_f = m.environments(f, Hf)
+    e_f = m.expectation_value(f, Hf, envs_f)
+    v0 = f.to_dense().cpu().numpy()
+    f = Cb.changebonds_finite(f, None, Cb.RandExpand(trunc_dim=2, seed=4), None)
+    assert max(f.bond_dims()) == 6
+    v1 = f.to_dense().cpu().numpy()
+    assert abs(abs(np.dot(v0, v1)) / (np.linalg.norm(v0) * np.linalg.norm(v1)) - 1.0) < 1e-12
+    assert abs(m.expectation_value(f, Hf, m.environments(f, Hf)) - e_f) < 1e-10
