@@ -1,15 +1,15 @@
 """Bond expansion and truncation (changebonds with OptimalExpand / SvdCut) on the HIP path.
 
 Mirrors (paths relative to the MPSKit.jl tree):
-  OptimalExpand  src/algorithms/changebonds/optimalexpand.jl:1-110 (Zauner-Stauber
+  OptimalExpand  src/algorithms/changebonds/optimalexpand.jl:37-117 (Zauner-Stauber
                  et al., PRB 97 (2018)): per bond, the two-site update dAC2(AC AR)
                  projected onto the null spaces of AL and AR, its dominant
                  ``trunc_dim`` singular directions added to the bond
   _expand!       src/algorithms/changebonds/changebonds.jl:59-84: AL -> [AL expansion;
                  0 0], AR -> [AR 0; expansion 0], C -> [C 0; 0 0] (the state is unchanged)
-  finite         src/algorithms/changebonds/optimalexpand.jl:112-140 (changebond! :right per
+  finite         src/algorithms/changebonds/optimalexpand.jl:119-198 (changebond! :right per
                  bond, left to right): AC -> [AC 0] re-gauged, AR -> [AR; added rows]
-  SvdCut         src/algorithms/changebonds/svdcut.jl: truncated SVD of every bond
+  SvdCut         src/algorithms/changebonds/svdcut.jl:21-41,98-120: truncated SVD of every bond
                  matrix C in the mixed-canonical gauge, right to left for a finite
                  MPS; for an infinite MPS AL[i] U, U^T AL[i+1] and a new gauge fix
 
@@ -32,7 +32,7 @@ from .vumps import InfiniteMPS
 
 @dataclass
 class RandExpand:
-    """RandExpand(; trunc = truncrank(trunc_dim)) (randexpand.jl): at most
+    """RandExpand(; trunc = truncrank(trunc_dim)) (randexpand.jl:29-35): at most
     ``trunc_dim`` random directions from the null spaces of AL and AR added to
     every bond (no Hamiltonian involved)."""
     trunc_dim: int = 1
@@ -138,7 +138,7 @@ def _keep(S, alg: SvdCut):
 
 
 def svdcut_finite(psi: FiniteMPS, alg: SvdCut) -> FiniteMPS:
-    """changebonds!(psi::FiniteMPS, SvdCut(...)) (svdcut.jl:22-32), in place:
+    """changebonds!(psi::FiniteMPS, SvdCut(...)) (svdcut.jl:32-41), in place:
     bonds L-2 .. 0 in turn, each bond matrix from an LQ of the center."""
     for b in range(psi.L - 2, -1, -1):
         psi.move_center(b + 1)
@@ -154,7 +154,7 @@ def svdcut_finite(psi: FiniteMPS, alg: SvdCut) -> FiniteMPS:
 
 
 def svdcut_infinite(psi: InfiniteMPS, alg: SvdCut) -> InfiniteMPS:
-    """changebonds(psi::InfiniteMPS, SvdCut(...)) (svdcut.jl:80-100): truncate each
+    """changebonds(psi::InfiniteMPS, SvdCut(...)) (svdcut.jl:98-120): truncate each
     bond in the singular basis of its C, then gauge-fix the new tensors."""
     N = psi.N
     A = [a.clone() for a in psi.AL]
@@ -179,7 +179,7 @@ def _complement_cols(M):
 
 
 def changebonds_finite(psi: FiniteMPS, H, alg, envs):
-    """changebonds!(psi::FiniteMPS, H, OptimalExpand(...), envs) (optimalexpand.jl:112-175):
+    """changebonds!(psi::FiniteMPS, H, OptimalExpand(...), envs) (optimalexpand.jl:119-198):
     every bond, left to right, grows by at most ``trunc_dim`` directions taken from
     the two-site update projected onto the complement of the current state; the
     state is unchanged (the new directions carry zero weight).  ``envs`` are the
@@ -196,7 +196,7 @@ def changebonds_finite(psi: FiniteMPS, H, alg, envs):
         NR = _complement_cols(T.fuse(right, 1).t()).t()
         k = min(alg.trunc_dim, NL.shape[1], NR.shape[0])
         if k > 0 and isinstance(alg, RandExpand):
-            # randexpand.jl: a randomised two-site update restricted to the complement
+            # randexpand.jl:79-120: a randomised two-site update restricted to the complement
             g2 = torch.randn(NL.shape[1], NR.shape[0], generator=gen, dtype=torch.float64, device=NL.device)
             nrm = float(torch.linalg.matrix_norm(g2))
         elif k > 0:
@@ -231,7 +231,7 @@ def changebonds_finite(psi: FiniteMPS, H, alg, envs):
 
 
 def randexpand_infinite(psi: InfiniteMPS, alg: RandExpand) -> InfiniteMPS:
-    """changebonds!(psi::InfiniteMPS, RandExpand(...)) (randexpand.jl:28-44):
+    """changebonds!(psi::InfiniteMPS, RandExpand(...)) (randexpand.jl:37-57):
     random isometries into the left null space of AL[i] and the right null space
     of AR[i+1]."""
     g = torch.Generator(device=psi.AL[0].device).manual_seed(alg.seed)

@@ -118,7 +118,7 @@ def _mpo_product(W1, W2):
 
 
 def variance(psi: FiniteMPS, H):
-    """<H^2> - <H>^2 for a finite MPS (variance(state::FiniteMPS, H), toolbox.jl:120-123):
+    """<H^2> - <H>^2 for a finite MPS (variance(state::FiniteMPS, H), toolbox.jl:125-128):
     <psi|H H|psi> through the product MPO with the HIP MPO transfers, <H> as energy."""
     from .algorithms import expectation_value
     from .environments import FiniteEnvironments
