@@ -3,7 +3,7 @@
 Mirrors (paths relative to the MPSKit.jl tree):
   ProjectionDerivativeOperator  src/algorithms/derivatives/projection_derivatives.jl:1-80
                                 (two-site projector derivative of |psi_i><psi_i|)
-  projection environments       src/environments/finite_envs.jl:84-95 (overlaps of the
+  projection environments       src/environments/finite_envs.jl:83-91 (overlaps of the
                                 optimised state with psi_i, MPS transfers)
   FiniteExcited / excitations   src/algorithms/excitation/dmrgexcitation.jl:18-70
                                 (DMRG on H + w sum_i |psi_i><psi_i|, perturbed start)
