@@ -64,3 +64,28 @@ def test_package_has_no_oracle_dependency():
     for p in (ROOT / "mpskit.jl_amd").rglob("*.py"):
         src = p.read_text()
         assert "oracle" not in re.sub(r"#.*", "", src).replace('"""', ""), p
+
+
+def test_integration_ccall_shims_match_the_header():
+    """Every Julia ccall shown in INTEGRATION.md passes as many arguments as the
+    header declares for that entry point."""
+    hdr = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "mpskit_hip.h").read_text(), flags=re.S)
+    decl = {m.group(1): len([a for a in m.group(2).split(",") if a.strip() and a.strip() != "void"])
+            for m in re.finditer(r"mk_status\s+(mk_\w+)\s*\(([^;]*?)\);", hdr, flags=re.S)}
+    text = (ROOT / "INTEGRATION.md").read_text()
+    checked = 0
+    for m in re.finditer(r"ccall\(\(:(mk_\w+),\s*\w+\),\s*\w+,\s*", text):
+        rest = text[m.end():]
+        assert rest.startswith("("), m.group(1)
+        depth, end = 0, 0
+        for k, ch in enumerate(rest):
+            depth += ch == "("
+            depth -= ch == ")"
+            if depth == 0:
+                end = k
+                break
+        types = [t for t in rest[1:end].split(",") if t.strip()]
+        assert m.group(1) in decl, m.group(1)
+        assert len(types) == decl[m.group(1)], (m.group(1), len(types), decl[m.group(1)])
+        checked += 1
+    assert checked >= 10
