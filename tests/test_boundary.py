@@ -71,7 +71,7 @@ def test_integration_ccall_shims_match_the_header():
     header declares for that entry point."""
     hdr = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "mpskit_hip.h").read_text(), flags=re.S)
     decl = {m.group(1): len([a for a in m.group(2).split(",") if a.strip() and a.strip() != "void"])
-            for m in re.finditer(r"mk_status\s+(mk_\w+)\s*\(([^;]*?)\);", hdr, flags=re.S)}
+            for m in re.finditer(r"\b(mk_\w+)\s*\(([^;()]*?)\);", hdr, flags=re.S)}
     text = (ROOT / "INTEGRATION.md").read_text()
     checked = 0
     for m in re.finditer(r"ccall\(\(:(mk_\w+),\s*\w+\),\s*\w+,\s*", text):
