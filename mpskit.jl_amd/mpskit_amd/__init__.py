@@ -35,3 +35,7 @@ __all__ += ["Davidson", "MLPPreconditioner", "davidson", "find_groundstate_batch
             "find_groundstate_u1", "set_charge_blocks"]
 __all__ += ["DMRG", "DMRG2", "AdaptiveKrylov", "Lanczos", "expectation_value", "find_groundstate", "FiniteEnvironments",
             "environments", "FiniteMPOHamiltonian", "FiniteMPS"]
+
+# algorithm modules beyond DMRG2 / VUMPS, reachable as mpskit_amd.<module>
+from . import approximate, boundary, changebonds, ed, excited, idmrg, observables, tdvp, vumps  # noqa: E402,F401
+
