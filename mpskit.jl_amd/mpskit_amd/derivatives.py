@@ -27,7 +27,7 @@ from typing import Iterable
 import torch
 
 from . import tensors as T
-from ._lib import BF16X3, FP32, FP64, PREC_NAMES, SpaceMismatch, check, load
+from ._lib import FP64, PREC_NAMES, SpaceMismatch, check, load
 from .context import default_context
 
 PrecisionArg = str | int

@@ -9,8 +9,6 @@ transfer applications of the HIP library (transfer.jl:125-130).
 """
 from __future__ import annotations
 
-import torch
-
 from . import tensors as T
 from .transfer import transfer_left, transfer_right
 
