@@ -195,6 +195,9 @@ def marek_gap(spectrum, tol_angle=0.1):
     epsilon = -log|lambda| of the first, delta the gap to the next at that angle."""
     spec = np.asarray(spectrum)
     spec = spec[np.abs(spec) < 1 - 1e-12]
+    if spec.size == 0:
+        raise ValueError("marek_gap: no eigenvalue below modulus 1 (request more eigenvalues, or the MPS "
+                         "is not injective)")
     spec = spec[np.argsort(-np.abs(spec))]
     angles = np.angle(spec)
     theta = angles[0]

@@ -184,3 +184,9 @@ def test_infinite_mpo_rejects_inconsistent_bonds():
         InfiniteMPO([np.zeros((2, 2, 2, 3))])            # right bond 3 != left bond 2 of the next site
     with pytest.raises(ValueError):
         InfiniteMPO([np.zeros((2, 2, 3, 2))])            # physical legs 2 and 3
+
+
+def test_marek_gap_needs_an_eigenvalue_below_one():
+    from mpskit_amd.observables import marek_gap
+    with pytest.raises(ValueError):
+        marek_gap(np.array([1.0, -1.0 + 1e-14]))
