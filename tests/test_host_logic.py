@@ -190,3 +190,17 @@ def test_marek_gap_needs_an_eigenvalue_below_one():
     from mpskit_amd.observables import marek_gap
     with pytest.raises(ValueError):
         marek_gap(np.array([1.0, -1.0 + 1e-14]))
+
+
+def test_argument_checks_before_any_device_work():
+    """correlator, approximate and dot reject bad arguments before touching a state."""
+    from types import SimpleNamespace
+    from mpskit_amd import approximate as Ap, observables as Ob
+    with pytest.raises(ValueError):
+        Ob.correlator(None, np.eye(2), np.eye(2), 3, [2, 5])        # a j not to the right of i
+    with pytest.raises(ValueError):
+        Ob.correlator(None, np.eye(2), np.eye(2), 3, [])
+    with pytest.raises(ValueError):
+        Ap.approximate(SimpleNamespace(L=6), None, SimpleNamespace(L=5), Ap.ApproximateDMRG2())
+    with pytest.raises(ValueError):
+        Ob.dot(SimpleNamespace(L=4), SimpleNamespace(L=5))
