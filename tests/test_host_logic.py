@@ -204,3 +204,40 @@ def test_argument_checks_before_any_device_work():
         Ap.approximate(SimpleNamespace(L=6), None, SimpleNamespace(L=5), Ap.ApproximateDMRG2())
     with pytest.raises(ValueError):
         Ob.dot(SimpleNamespace(L=4), SimpleNamespace(L=5))
+
+
+@pytest.fixture
+def cpu_layout(monkeypatch):
+    """The Krylov solvers' vector layout helper keeps CPU tensors on the CPU here."""
+    from mpskit_amd import tensors as T
+    monkeypatch.setattr(T, "as_colmajor", lambda t: t)
+    return T
+
+
+def test_krylov_solvers_against_dense_linear_algebra(cpu_layout):
+    """arnoldi_dominant, gmres, exponentiate and lanczos_min on dense matrices
+    (1-D vectors, where the column-major layout is trivial) against numpy/scipy."""
+    import torch
+    from scipy.linalg import expm
+    from mpskit_amd.excited import lanczos_min
+    from mpskit_amd.tdvp import exponentiate
+    from mpskit_amd.vumps import arnoldi_dominant, gmres
+    rng = np.random.default_rng(21)
+    n = 120
+    P = torch.from_numpy(rng.random((n, n)))                       # positive: real Perron eigenvalue
+    lam, v = arnoldi_dominant(lambda x: P @ x, torch.from_numpy(rng.random(n)), tol=1e-12, relative=True)
+    ev = np.linalg.eigvals(P.numpy())
+    assert abs(lam - ev[np.argmax(np.abs(ev))].real) < 1e-9 * abs(lam)
+    assert float(torch.linalg.vector_norm(P @ v - lam * v)) < 1e-9 * abs(lam)
+    A = torch.from_numpy(np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.2)
+    b = torch.from_numpy(rng.standard_normal(n))
+    x, info = gmres(lambda y: A @ y, b, torch.zeros(n, dtype=torch.float64), 1e-12)
+    assert info["converged"] and float(torch.linalg.vector_norm(A @ x - b)) < 1e-11
+    S = rng.standard_normal((n, n))
+    S = torch.from_numpy((S + S.T) / np.sqrt(n))
+    x0 = rng.standard_normal(n)
+    y, _ = exponentiate(lambda z: S @ z, -0.7, torch.from_numpy(x0), tol=1e-13)
+    ref = expm(-0.7 * S.numpy()) @ x0
+    assert np.linalg.norm(y.numpy() - ref) < 1e-11 * np.linalg.norm(ref)
+    lam0, u, _ = lanczos_min(lambda z: S @ z, torch.from_numpy(rng.standard_normal(n)), tol=1e-11)
+    assert abs(lam0 - np.linalg.eigvalsh(S.numpy())[0]) < 1e-9
