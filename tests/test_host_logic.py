@@ -241,3 +241,18 @@ def test_krylov_solvers_against_dense_linear_algebra(cpu_layout):
     assert np.linalg.norm(y.numpy() - ref) < 1e-11 * np.linalg.norm(ref)
     lam0, u, _ = lanczos_min(lambda z: S @ z, torch.from_numpy(rng.standard_normal(n)), tol=1e-11)
     assert abs(lam0 - np.linalg.eigvalsh(S.numpy())[0]) < 1e-9
+
+
+def test_complement_columns_span_the_orthogonal_complement():
+    """The null-space helper of the finite expansions: orthonormal columns,
+    orthogonal to range(M), of dimension m - rank(M) (rank-deficient M too)."""
+    import torch
+    from mpskit_amd.changebonds import _complement_cols
+    g = torch.Generator().manual_seed(2)
+    for m, n, r in ((12, 5, 5), (12, 5, 3), (8, 8, 8)):
+        M = torch.randn(m, r, generator=g, dtype=torch.float64) @ torch.randn(r, n, generator=g,
+                                                                          dtype=torch.float64)
+        N = _complement_cols(M)
+        assert N.shape == (m, m - r)
+        assert float(torch.linalg.matrix_norm(N.t() @ N - torch.eye(m - r, dtype=torch.float64))) < 1e-12
+        assert float(torch.linalg.matrix_norm(N.t() @ M)) < 1e-12 * float(torch.linalg.matrix_norm(M))
