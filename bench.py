@@ -314,13 +314,15 @@ def batched_workload(args):
     print(json.dumps(out), flush=True)
 
 
-U1_CENTER = [8, 8, 8, 16, 40, 64, 112, 168, 216, 256, 256, 256, 216, 168, 112, 64, 40, 16, 8, 8, 8]
+# 2048 states over 21 charge sectors, a discretised Gaussian in the charge (sigma ~ 3 sectors)
+U1_CENTER = [1, 3, 8, 18, 37, 68, 112, 165, 218, 258, 272, 258, 218, 165, 112, 68, 37, 18, 8, 3, 1]
 
 
 def u1_workload(args):
     """U(1) XXZ DMRG2 (L = 64, Sz = 0, D = 2048 over 21 charge sectors with a
-    discretised Gaussian profile in multiples of 8): sweep time and applies in
-    float64 and in the mixed path, energies of both."""
+    discretised Gaussian profile; interior sectors carry zero padding states up
+    to a multiple of 8): sweep time and applies in float64 and in the mixed
+    path, energies of both."""
     import torch
     import mpskit_amd as mk
     L, nup = args.L if args.L != 48 else 64, None

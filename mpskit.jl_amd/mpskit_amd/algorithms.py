@@ -227,10 +227,14 @@ def gram_eigh_grouped(mats):
 
 def truncated_split_batched(Ms, direction, trunc_dim, trunc_err):
     """``truncated_split(..., method="eigh")`` of many matrices at once, the Gram
-    matrices decomposed by :func:`gram_eigh_grouped`."""
+    matrices decomposed by :func:`gram_eigh_grouped`.  ``trunc_dim`` is one bond
+    cap for all matrices or a list with one cap per matrix."""
     sides = [M if direction == "right" else M.t() for M in Ms]
-    return [_split_from_gram(M, direction, lam, Q, trunc_dim, trunc_err)
-            for M, (lam, Q) in zip(Ms, gram_eigh_grouped(sides))]
+    caps = list(trunc_dim) if isinstance(trunc_dim, (list, tuple)) else [trunc_dim] * len(Ms)
+    if len(caps) != len(Ms):
+        raise ValueError("truncated_split_batched: one trunc_dim per matrix")
+    return [_split_from_gram(M, direction, lam, Q, k, trunc_err)
+            for M, (lam, Q), k in zip(Ms, gram_eigh_grouped(sides), caps)]
 
 
 def gauge2(psi: FiniteMPS, pos, direction, ac2, trunc_dim, trunc_err, method="auto", split=None):
@@ -438,7 +442,8 @@ def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None
     sweep the energies, the wall time and the applies of the slowest instance.
     ``on_bond(sweep, pos, direction, ops, xs)``, if given, sees every bond's
     prepared operators and start vectors before the batched solve (its time is
-    excluded from the sweep time).
+    excluded from the sweep time).  ``alg.trunc_dim`` may be a list with one
+    bond cap per instance (uneven instances of a parameter sweep).
     """
     from .derivatives import BatchedDerivative
     solver = solver or Lanczos(tol=1e-10, krylovdim=20, maxiter=100)
@@ -446,6 +451,9 @@ def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None
     L = psis[0].L
     if any(p.L != L for p in psis) or len(Hs) != n:
         raise ValueError("find_groundstate_batched: instances must be chains of one length")
+    caps = list(alg.trunc_dim) if isinstance(alg.trunc_dim, (list, tuple)) else [alg.trunc_dim] * n
+    if len(caps) != n:
+        raise ValueError("find_groundstate_batched: one trunc_dim per instance")
     envs = [FiniteEnvironments(p, H) for p, H in zip(psis, Hs)]
     precs = ("fp64",) if alg.precision == "fp64" else ("fp64", alg.precision)
     bonds = [(p, "right") for p in range(L - 1)] + [(p, "left") for p in range(L - 3, -1, -1)]
@@ -478,10 +486,9 @@ def find_groundstate_batched(psis, Hs, alg: DMRG2, solver: Lanczos | None = None
             if alg.gauge == "svd":
                 splits = [None] * n
             else:
-                splits = truncated_split_batched([T.fuse(v, 2) for v in vecs], direction, alg.trunc_dim,
-                                                 alg.trunc_err)
-            for psi, v, sp in zip(psis, vecs, splits):
-                gauge2(psi, pos, direction, v, alg.trunc_dim, alg.trunc_err, alg.gauge, split=sp)
+                splits = truncated_split_batched([T.fuse(v, 2) for v in vecs], direction, caps, alg.trunc_err)
+            for psi, v, sp, cap in zip(psis, vecs, splits, caps):
+                gauge2(psi, pos, direction, v, cap, alg.trunc_err, alg.gauge, split=sp)
             Bt.free()
             for op in ops:
                 op.free()

@@ -70,19 +70,45 @@ def xxz_bulk(delta=1.0):
     return _jordan(2, [0.5 * SPLUS, 0.5 * SMINUS, delta * SZ], [SMINUS, SPLUS, SZ], np.zeros((2, 2)))
 
 
+def heisenberg_range_bulk(js):
+    """H = sum_r js[r-1] sum_i S_i.S_{i+r}, r = 1..R  (levels 1,3,..,3,1 -> w = 3R + 2, d = 2).
+
+    Channel block j (j = 0..R-1, channels 3j+1..3j+3) carries a pair term that
+    still has R-j sites to go: the start operator enters block j with coupling
+    js[R-1-j], an identity in the A block passes block j on to block j+1, and
+    only the last block ends the term.  Couplings only go to higher channels
+    (upper triangular), the level order the environment fixed points rely on.
+    For R >= 3 the A blocks of two neighbouring sites connect (block 0 -> 1 -> 2),
+    which is the continuing-continuing term of a two-site derivative
+    (hamiltonian_derivatives.jl:243-270, :352-362)."""
+    R = len(js)
+    a = [SPLUS / np.sqrt(2.0), SMINUS / np.sqrt(2.0), SZ]
+    b = [SMINUS / np.sqrt(2.0), SPLUS / np.sqrt(2.0), SZ]
+    C, B, A = [], [], []
+    for j in range(R):
+        C += [js[R - 1 - j] * o for o in a]
+        B += (b if j == R - 1 else [np.zeros((2, 2))] * 3)
+        if j + 1 < R:
+            A += [(1 + 3 * j + k, 4 + 3 * j + k, np.eye(2)) for k in range(3)]
+    return _jordan(2, C, B, np.zeros((2, 2)), A)
+
+
 def j1j2_bulk(j1=1.0, j2=0.24):
     """H = j1 sum S_i.S_{i+1} + j2 sum S_i.S_{i+2}  (levels 1,3,3,1 -> w = 8, d = 2).
 
     Channels 1-3 start a next-nearest-neighbour term, which an identity in the A
     block passes to channels 4-6; channels 4-6 also start the nearest-neighbour
-    term, and both end there.  Couplings only go to higher channels (upper
-    triangular), the level order the environment fixed points rely on."""
-    a = [SPLUS / np.sqrt(2.0), SMINUS / np.sqrt(2.0), SZ]
-    b = [SMINUS / np.sqrt(2.0), SPLUS / np.sqrt(2.0), SZ]
-    C = [j2 * o for o in a] + [j1 * o for o in a]
-    B = [np.zeros((2, 2))] * 3 + b
-    A = [(1 + k, 4 + k, np.eye(2)) for k in range(3)]
-    return _jordan(2, C, B, np.zeros((2, 2)), A)
+    term, and both end there."""
+    return heisenberg_range_bulk([j1, j2])
+
+
+def exp_ising_bulk(J=1.0, lam=0.5, g=1.0):
+    """H = sum_{i<j} J lam^(j-i-1) sigma^z_i sigma^z_j - g sum sigma^x_i  (w = 3, d = 2).
+
+    The single middle channel feeds itself through A = lam * 1, so the A blocks of
+    any two neighbouring bulk sites connect: an infinite-range (exponentially
+    decaying) interaction in three channels."""
+    return _jordan(2, [SIGMA_Z], [J * SIGMA_Z], -g * SIGMA_X, [(1, 1, lam * ID2)])
 
 
 def tfim(L, g=1.0):
@@ -95,6 +121,14 @@ def xxz(L, delta=1.0):
 
 def j1j2(L, j1=1.0, j2=0.24):
     return finite_chain(j1j2_bulk(j1, j2), L)
+
+
+def heisenberg_range(L, js):
+    return finite_chain(heisenberg_range_bulk(js), L)
+
+
+def exp_ising(L, J=1.0, lam=0.5, g=1.0):
+    return finite_chain(exp_ising_bulk(J, lam, g), L)
 
 
 def classical_ising(beta=float(np.log(1.0 + np.sqrt(2.0)) / 2)):

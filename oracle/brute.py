@@ -64,6 +64,23 @@ def dense_j1j2(L, j1, j2):
     return j1 * bond_sum(L, t, 1) + j2 * bond_sum(L, t, 2)
 
 
+def dense_heisenberg_range(L, js):
+    """H = sum_r js[r-1] sum_i S_i.S_{i+r} for an open chain."""
+    t = [(0.5, SP, SM), (0.5, SM, SP), (1.0, SZ, SZ)]
+    H = sp.csr_matrix((2 ** L, 2 ** L))
+    for r, j in enumerate(js, start=1):
+        H = H + j * bond_sum(L, t, r)
+    return H
+
+
+def dense_exp_ising(L, J, lam, g):
+    """H = sum_{i<j} J lam^(j-i-1) sigma^z_i sigma^z_j - g sum sigma^x_i (open chain)."""
+    H = field_sum(L, -g, PX)
+    for r in range(1, L):
+        H = H + J * lam ** (r - 1) * bond_sum(L, [(1.0, PZ, PZ)], r)
+    return H
+
+
 def mpo_to_dense(Ws):
     """Contract a finite MPO (W[g,b,h,k], boundary bonds of dimension 1) to a matrix."""
     T = Ws[0][0]                                  # (b, h, k)

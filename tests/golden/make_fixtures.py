@@ -23,7 +23,11 @@ from oracle import oracle as O  # noqa: E402
 from oracle import brute as B  # noqa: E402
 from mpskit_amd import models as M  # noqa: E402
 
-BULK = {"tfim": M.tfim_bulk(0.7), "xxz": M.xxz_bulk(0.6), "j1j2": M.j1j2_bulk(1.0, 0.24)}
+BULK = {"tfim": M.tfim_bulk(0.7), "xxz": M.xxz_bulk(0.6), "j1j2": M.j1j2_bulk(1.0, 0.24),
+        # connected A blocks: the two-site Jordan operator carries its AA term
+        "heis3": M.heisenberg_range_bulk([1.0, 0.45, 0.3]), "expising": M.exp_ising_bulk(0.8, 0.6, 0.9)}
+CHAIN = {"tfim": lambda L: M.tfim(L, 0.7), "xxz": lambda L: M.xxz(L, 0.6), "j1j2": lambda L: M.j1j2(L, 1.0, 0.24),
+         "heis3": lambda L: M.heisenberg_range(L, [1.0, 0.45, 0.3]), "expising": lambda L: M.exp_ising(L, 0.8, 0.6, 0.9)}
 
 
 def unit(rng, *shape):
@@ -66,7 +70,7 @@ def cases():
             vr3 = unit(rng, Dc, w, Di)
             out[f"trmpo_{k}"] = dict(v=vr3, O=W, A=A, Ab=Ab, y=O.transfer_right_mpo(vr3, W, A, Ab))
         # a boundary pair: first two sites of a finite chain (w_l = 1)
-        Ws = {"tfim": M.tfim(6, 0.7), "xxz": M.xxz(6, 0.6), "j1j2": M.j1j2(6, 1.0, 0.24)}[name]
+        Ws = CHAIN[name](6)
         rng = np.random.default_rng(77 + mi)
         psi = B.random_mps(6, 2, 8, rng)
         AL, AR = B.left_canonical(psi), B.right_canonical(psi)

@@ -118,6 +118,30 @@ def test_jordan_ac2_odd_sizes_vs_oracle(mk, D):
         assert e < TOL[p], (p, e)
 
 
+@pytest.mark.parametrize("model,D", [("heis3", 37), ("heis3", 130), ("expising", 45), ("expising", 300)])
+def test_jordan_ac2_aa_term_vs_oracle(mk, model, D):
+    """Connected A blocks (range 3, exponential decay): the HIP Jordan operator's AA
+    term over _connected_channels (hamiltonian_derivatives.jl:243-270, :352-362) plus
+    its CA / AB folds, against the oracle in every precision."""
+    rng = np.random.default_rng(D + 7)
+    M = __import__("mpskit_amd").models
+    W = M.heisenberg_range_bulk([1.0, 0.45, 0.3]) if model == "heis3" else M.exp_ising_bulk(0.8, 0.6, 0.9)
+    w = W.shape[0]
+    GL, GR = rng.standard_normal((D, w, D)) / D, rng.standard_normal((D, w, D)) / D
+    x = rng.standard_normal((D, 2, D, 2))
+    T = O.jordan_ac2_prepare(GL, W, W, GR)
+    assert T["AA"] is not None
+    y_ref = O.jordan_ac2_apply(T, x)
+    print("jordan ac2 AA", model, "D", D, flush=True)
+    op = mk.prepare_operator(mk.JordanMPO_AC2_Hamiltonian(dev(mk, GL), dev(mk, W), dev(mk, W), dev(mk, GR)),
+                             precisions=PRECS)
+    xd = dev(mk, x)
+    for p in PRECS:
+        e = rel(host(mk, op(xd, precision=p)), y_ref)
+        print(f"  {p}: {e:.2e}", flush=True)
+        assert e < TOL[p], (p, e)
+
+
 @pytest.mark.parametrize("D,w", [(45, 3), (129, 5), (70, 8)])
 def test_generic_ops_odd_sizes_vs_oracle(mk, D, w):
     rng = np.random.default_rng(D + w)

@@ -18,6 +18,10 @@ CHAINS = {
     "tfim": (lambda: M.tfim(L, 0.7), lambda: B.dense_tfim(L, 0.7)),
     "xxz": (lambda: M.xxz(L, 0.6), lambda: B.dense_xxz(L, 0.6)),
     "j1j2": (lambda: M.j1j2(L, 1.0, 0.3), lambda: B.dense_j1j2(L, 1.0, 0.3)),
+    # range 3 and infinite range: the A blocks of neighbouring sites connect, so the
+    # Jordan two-site operator carries the AA term (hamiltonian_derivatives.jl:352-362)
+    "heis3": (lambda: M.heisenberg_range(L, [1.0, 0.45, 0.3]), lambda: B.dense_heisenberg_range(L, [1.0, 0.45, 0.3])),
+    "expising": (lambda: M.exp_ising(L, 0.8, 0.6, 0.9), lambda: B.dense_exp_ising(L, 0.8, 0.6, 0.9)),
 }
 
 
@@ -91,6 +95,103 @@ def test_jordan_applies_vs_projected_dense_H(name):
         P = B.isometry_ac(AL, AR, j, 2)
         T = O.jordan_ac_prepare(GL[j], Ws[j], GR[j + 1])
         assert rel(O.jordan_ac_apply(T, x).reshape(-1, order="F"), P.T @ H @ P @ x.reshape(-1, order="F")) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["heis3", "expising"])
+def test_jordan_aa_term_connected_channels(name):
+    """The continuing-continuing term over ``_connected_channels``
+    (hamiltonian_derivatives.jl:243-270, :352-362): on every bulk bond of a range-3
+    chain the channel sets are the pass-through blocks, the AA term is present, and
+    dropping it (a plausible omission) breaks P^T H P, so the projected-H pin above
+    covers the AA branch."""
+    rng = np.random.default_rng(13)
+    Ws, H = CHAINS[name][0](), CHAINS[name][1]().toarray()
+    psi = B.random_mps(L, 2, 6, rng)
+    (AL, AR, _, _), GL, GR = envs(Ws, psi, psi)
+    n_aa = 0
+    for j in range(L - 1):
+        T = O.jordan_ac2_prepare(GL[j], Ws[j], Ws[j + 1], GR[j + 2])
+        b1, b2 = O.jordan_split(Ws[j]), O.jordan_split(Ws[j + 1])
+        ch = O._connected_channels(b1["A"], b2["A"])
+        bulk = 1 <= j and j + 1 <= L - 2
+        assert (ch is not None) == bulk and (T["AA"] is not None) == bulk
+        if not bulk:
+            continue
+        n_aa += 1
+        rows, mids, cols = ch
+        if name == "heis3":          # block 0 -> block 1 -> block 2 of the mid channels
+            assert list(rows) == [0, 1, 2] and list(mids) == [3, 4, 5] and list(cols) == [6, 7, 8]
+        else:                        # the single channel feeds itself
+            assert list(rows) == [0] and list(mids) == [0] and list(cols) == [0]
+        x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 2].shape[0], 2))
+        P = B.isometry_ac2(AL, AR, j, 2)
+        yb = P.T @ H @ P @ x.reshape(-1, order="F")
+        assert rel(O.jordan_ac2_apply(T, x).reshape(-1, order="F"), yb) < 1e-13
+        no_aa = dict(T, AA=None)
+        assert rel(O.jordan_ac2_apply(no_aa, x).reshape(-1, order="F"), yb) > 1e-3
+    assert n_aa == L - 3
+
+
+def test_jordan_aa_term_on_random_environments():
+    """AA with i.i.d. environments (the reference benchmark's setup, e.g. the
+    heisenberg_nnn / coulomb specs): Jordan == generic with identity corner levels."""
+    rng = np.random.default_rng(14)
+    for W in (M.heisenberg_range_bulk([1.0, 0.4, 0.25]), M.exp_ising_bulk(1.0, 0.7, 0.5)):
+        w, D = W.shape[0], 5
+        GL, GR = rng.standard_normal((D, w, D)), rng.standard_normal((D, w, D))
+        GL[:, 0, :] = np.eye(D)
+        GR[:, -1, :] = np.eye(D)
+        T = O.jordan_ac2_prepare(GL, W, W, GR)
+        assert T["AA"] is not None
+        x = rng.standard_normal((D, 2, D, 2))
+        assert rel(O.jordan_ac2_apply(T, x), O.ac2_apply_generic(GL, W, W, GR, x)) < 1e-13
+
+
+def test_jordan_nn_gemm_form_equals_jordan_apply():
+    """The two-GEMM form of the nearest-neighbour operator (the timed CPU baseline)
+    equals the term-by-term apply; it refuses operators with A blocks."""
+    rng = np.random.default_rng(15)
+    for W in (M.tfim_bulk(0.7), M.xxz_bulk(0.6)):
+        w, Dl, Dr = W.shape[0], 6, 7
+        GL, GR = rng.standard_normal((Dl, w, Dl)), rng.standard_normal((Dr, w, Dr))
+        T = O.jordan_ac2_prepare(GL, W, W, GR)
+        ICm, BEm, CB = O.jordan_ac2_nn_matrices(T, Dl, 2, Dr, 2)
+        x = rng.standard_normal((Dl, 2, Dr, 2))
+        assert rel(O.jordan_ac2_apply_nn(ICm, BEm, CB, x), O.jordan_ac2_apply(T, x)) < 1e-13
+    Tj = O.jordan_ac2_prepare(GL[:, :1].repeat(8, 1), M.j1j2_bulk(), M.j1j2_bulk(), GR[:, :1].repeat(8, 1))
+    with pytest.raises(AssertionError):
+        O.jordan_ac2_nn_matrices(Tj, Dl, 2, Dr, 2)
+
+
+def test_mps_transfers_vs_kronecker_matrices():
+    """transfer_left/right for MPS tensors (transfer.jl:18-55) against the transfer
+    matrix built from Kronecker products E = sum_h kron(Ab_h, A_h) acting on vec(v),
+    with bra != ket and rectangular bonds; a right isometry is a right fixed point."""
+    rng = np.random.default_rng(16)
+    Dm, Dn, Da, Dc = 4, 5, 3, 6
+    A, Ab = rng.standard_normal((Dn, 2, Dc)), rng.standard_normal((Dm, 2, Da))
+    # left: y[a,c] = sum v[m,n] A[n,h,c] Ab[m,h,a]  ->  y = sum_h Ab_h^T v A_h
+    v = rng.standard_normal((Dm, Dn))
+    y = sum(Ab[:, h, :].T @ v @ A[:, h, :] for h in range(2))
+    assert rel(O.transfer_left_mps(v, A, Ab), y) < 1e-13
+    E = sum(np.kron(A[:, h, :].T, Ab[:, h, :].T) for h in range(2))      # acts on vec_F(v)
+    assert rel(E @ v.reshape(-1, order="F"), y.reshape(-1, order="F")) < 1e-13
+    # right: y[a,b] = sum A[a,h,i] Ab[b,h,j] v[i,j]  ->  y = sum_h A_h v Ab_h^T
+    u = rng.standard_normal((Dc, Da))
+    yr = sum(A[:, h, :] @ u @ Ab[:, h, :].T for h in range(2))
+    assert rel(O.transfer_right_mps(u, A, Ab), yr) < 1e-13
+    Er = sum(np.kron(Ab[:, h, :], A[:, h, :]) for h in range(2))
+    assert rel(Er @ u.reshape(-1, order="F"), yr.reshape(-1, order="F")) < 1e-13
+    # duality <v, T_R(u)> = <T_L(v), u> (the two are each other's adjoint)
+    lhs = np.sum(v.T * O.transfer_right_mps(u, A, Ab))
+    rhs = np.sum(O.transfer_left_mps(v, A, Ab) * u.T)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # a right isometry (sum_h A_h A_h^T = 1) is a fixed point of transfer_right
+    D = 5
+    Q, _ = np.linalg.qr(rng.standard_normal((D * 2, D)))
+    Ar = Q.T.reshape(D, 2, D, order="F")
+    assert rel(O.transfer_right_mps(np.eye(D), Ar, Ar), np.eye(D)) < 1e-13
+    assert rel(O.transfer_right_mps(np.eye(D), Ar.transpose(2, 1, 0), Ar.transpose(2, 1, 0)), np.eye(D)) > 1e-3
 
 
 def test_jordan_uses_identity_corner_levels():
