@@ -122,6 +122,44 @@ mk_status mk_prepare_ac2_jordan(mk_ctx ctx, const double* GL, const double* W1,
                                 int64_t wl, int64_t d1, int64_t wm, int64_t d2,
                                 int64_t wr, int64_t Dr, int precisions, mk_op* out);
 
+/* ---- Jordan operators from their constructed terms --------------------------
+ * The reference builds a JordanMPO_AC2_Hamiltonian / JordanMPO_AC_Hamiltonian
+ * (the constructor contracts the environments with the MPO blocks) and then
+ * calls the one-argument prepare_operator!!(H) on it (hamiltonian_derivatives.jl
+ * :206-217 and :44-55).  These entry points replace that prepare_operator!!:
+ * they take the constructed terms, device arrays in the layouts of the struct
+ * fields (NULL = `missing`), fold them as prepare_operator!! does and return
+ * a prepared operator for mk_apply / mk_eigsolve / batches.
+ *
+ * Two-site (hamiltonian_derivatives.jl:168-200 fields, :272-369 constructor,
+ * :371-468 prepare):
+ *   II[c,i] (Dr x Dr), IC[e,c,j,i] (d2 x Dr x d2 x Dr), ID[e,j] (d2 x d2),
+ *   CB[b,e,h,j] (d1 x d2 x d1 x d2), CA[b,e,c,h,j,i] (d1 x d2 x Dr x d1 x d2 x Dr),
+ *   AB[a,b,e,f,h,j] (Dl x d1 x d2 x Dl x d1 x d2), BE[a,b,f,h] (Dl x d1 x Dl x d1),
+ *   DE[b,h] (d1 x d1), EE[a,f] (Dl x Dl);
+ *   AA, the MPO_AC2_Hamiltonian over the connected channels (:352-362), as its
+ *   four dense tensors: AA_GL (Dl x aa_wl x Dl), AA_W1 (aa_wl x d1 x d1 x aa_wm),
+ *   AA_W2 (aa_wm x d2 x d2 x aa_wr), AA_GR (Dr x aa_wr x Dr).
+ * One-site (hamiltonian_derivatives.jl:15-32 fields, :57-107 constructor,
+ * :109-157 prepare):
+ *   D[b,h] (d x d), I[i,c] (Dr x Dr), E[a,f] (Dl x Dl), C[b,c,h,i] (d x Dr x d x Dr),
+ *   B[a,b,f,h] (Dl x d x Dl x d);  A, the MPO_AC_Hamiltonian over the middle
+ *   channels: A_GL (Dl x a_wl x Dl), A_W (a_wl x d x d x a_wr), A_GR (Dr x a_wr x Dr). */
+typedef struct mk_jordan_ac2_terms {
+  const double *II, *IC, *ID, *CB, *CA, *AB, *BE, *DE, *EE;
+  const double *AA_GL, *AA_W1, *AA_W2, *AA_GR;
+  int64_t aa_wl, aa_wm, aa_wr;
+} mk_jordan_ac2_terms;
+typedef struct mk_jordan_ac_terms {
+  const double *D, *I, *E, *C, *B;
+  const double *A_GL, *A_W, *A_GR;
+  int64_t a_wl, a_wr;
+} mk_jordan_ac_terms;
+mk_status mk_prepare_ac2_jordan_terms(mk_ctx ctx, const mk_jordan_ac2_terms* terms, int64_t Dl,
+                                      int64_t d1, int64_t Dr, int64_t d2, int precisions, mk_op* out);
+mk_status mk_prepare_ac_jordan_terms(mk_ctx ctx, const mk_jordan_ac_terms* terms, int64_t Dl, int64_t d,
+                                     int64_t Dr, int precisions, mk_op* out);
+
 /* ---- U(1) charge blocks ------------------------------------------------------
  * Counterpart of TensorKit's block-sparse U(1) storage under the same
  * derivative operators (the reference applies one GEMM per fusion-tree block,

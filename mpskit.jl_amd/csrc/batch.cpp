@@ -80,6 +80,13 @@ struct Batch {
   }
   int64_t xoff(int i) const { return off[i] * 8; }
   mk_status ensure_plan(int prec);
+  // wait for the batch's work on its own stream and on the context's stream
+  // (never device-wide: other contexts' streams keep running)
+  mk_status wait_streams() {
+    MK_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    if (stream) MK_CHECK_HIP(hipStreamSynchronize(stream));
+    return MK_OK;
+  }
   mk_status apply(const double* x, double* y, int prec, hipStream_t st);
   mk_status krylov_step(int j, int prec, hipStream_t st);
 };
@@ -200,7 +207,7 @@ mk_status Batch::ensure_plan(int prec) {
   }
   pp.scratch_bytes = sbytes;
   if (sbytes > 0 && (int64_t)scratch.bytes < sbytes) {
-    MK_CHECK_HIP(hipDeviceSynchronize());
+    MK_TRY(wait_streams());
     drop_graphs();
     scratch.release();
     MK_TRY(scratch.alloc((size_t)sbytes));
@@ -391,7 +398,7 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
   const bool graphs = use_graph && b->grouped;
   // buffers (addresses are part of captured graphs: reallocation drops them)
   if (b->ks < ks || !b->Q.ptr) {
-    MK_CHECK_HIP(hipDeviceSynchronize());
+    MK_TRY(b->wait_streams());
     b->drop_graphs();
     b->Q.release();
     b->coef.release();

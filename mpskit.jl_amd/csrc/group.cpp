@@ -129,19 +129,19 @@ void autotune_stats(int* decisions, int* hits) {
   if (hits) *hits = g_hits;
 }
 
-mk_status GroupPlan::build(int k) {
+mk_status GroupPlan::build(int k, hipStream_t launched_on) {
   MK_REQUIRE(tile_valid(prec, k), MK_BAD_ARG, "grouped gemm: tile kind %d invalid for precision %d", k, prec);
   std::vector<GemmProb> host;
   int total = 0;
   MK_TRY(fill(specs, prec, k, host, total));
   const size_t bytes = std::max<size_t>(host.size(), 1) * sizeof(GemmProb);
+  // launches of this plan (all on `launched_on`) may still be reading the
+  // array: wait for that stream only before replacing or rewriting it
+  if (dev.ptr && launched_on) MK_CHECK_HIP(hipStreamSynchronize(launched_on));
   if (dev.bytes < bytes) {
-    if (dev.ptr) MK_CHECK_HIP(hipDeviceSynchronize());
     dev.release();
     MK_TRY(dev.alloc(bytes));
   }
-  // launches of this plan may still be reading the array: wait before rewriting it
-  if (dev.ptr) MK_CHECK_HIP(hipDeviceSynchronize());
   if (!host.empty()) MK_CHECK_HIP(hipMemcpy(dev.ptr, host.data(), host.size() * sizeof(GemmProb), hipMemcpyHostToDevice));
   kind = k;
   nprob = (int)host.size();
@@ -253,7 +253,7 @@ mk_status group_plan(Ctx* ctx, const ProbSpec* probs, int n, int prec, GemmShape
   float best_ms = 1e30f;
   for (int k = TK_64; k <= TK_256; ++k) {
     if (!tile_valid(prec, k)) continue;
-    mk_status s = out->build(k);
+    mk_status s = out->build(k, st);
     if (s == MK_OK) s = out->run(base, st);  // warm-up
     if (s == MK_OK && hipEventRecord(e0, st) != hipSuccess) s = MK_HIP_ERROR;
     for (int r = 0; r < 3 && s == MK_OK; ++r) s = out->run(base, st);
@@ -272,13 +272,12 @@ mk_status group_plan(Ctx* ctx, const ProbSpec* probs, int n, int prec, GemmShape
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  MK_CHECK_HIP(hipStreamSynchronize(st));
   {
     std::lock_guard<std::mutex> lk(g_mu);
     g_cache[key] = best;
     ++g_decisions;
   }
-  return out->build(best);
+  return out->build(best, st);
 }
 
 }  // namespace mk

@@ -31,7 +31,9 @@ struct GroupPlan {
   double flops = 0.0;
   std::vector<ProbSpec> specs;  // problems in launch order (largest per-tile work first)
   DevBuf dev;                   // GemmProb array
-  mk_status build(int kind);    // fill and upload the descriptors for one tile kind
+  // fill and upload the descriptors for one tile kind; a rebuild first waits for
+  // the stream the plan's earlier launches went to
+  mk_status build(int kind, hipStream_t launched_on = nullptr);
   mk_status run(const void* const base[NBASE], hipStream_t st) const;
 };
 

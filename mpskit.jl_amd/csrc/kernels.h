@@ -108,4 +108,27 @@ mk_status k_nn_prologue(const double* P, const double* x, int64_t Dl, int64_t Dr
                         double* y, int split, void* xa_hi, void* xa_lo, int64_t ka, void* xb_hi, void* xb_lo,
                         int64_t kb, hipStream_t st);
 
+
+// ---- Jordan operator matrices from the constructed terms ------------------
+// The terms are arrays in the layouts of the reference's JordanMPO_AC2 /
+// JordanMPO_AC_Hamiltonian fields (null = missing); each routine writes one
+// GEMM operand with the terms prepare_operator!! folds into it.
+// ICm[(i,j),(c,e)] = IC[e,c,j,i] + ID[e,j] d_ci + d_ej II[c,i]          (Dr*d2 square)
+mk_status k_asm_ic(const double* IC, const double* ID, const double* II, int64_t Dr, int d2, double* out,
+                   hipStream_t st);
+// BEm[(a,b),(f,h)] = BE[a,b,f,h] + d_af DE[b,h] + EE[a,f] d_bh           (Dl*d1 square)
+mk_status k_asm_be(const double* BE, const double* DE, const double* EE, int64_t Dl, int d1, double* out,
+                   hipStream_t st);
+// CAm[(h,i,j),(b,c,e)] = CA[b,e,c,h,j,i] + CB[b,e,h,j] d_ci
+//                        + d_bh (IC[e,c,j,i] + ID[e,j] d_ci + d_ej II[c,i])   (d1*Dr*d2 square)
+mk_status k_asm_ca(const double* CA, const double* CB, const double* IC, const double* ID, const double* II,
+                   int64_t Dr, int d1, int d2, double* out, hipStream_t st);
+// ABm_(e,j)[(a,b),(f,h)] = AB[a,b,e,f,h,j] + CB[b,e,h,j] d_af
+//                        + d_ej (BE[a,b,f,h] + d_af DE[b,h] + EE[a,f] d_bh)   blocks (e,j) at e + d2*j
+mk_status k_asm_ab(const double* AB, const double* CB, const double* BE, const double* DE, const double* EE,
+                   int64_t Dl, int d1, int d2, double* out, hipStream_t st);
+// C'[(h,i),(b,c)] = C[b,c,h,i] + D[b,h] d_ic + d_bh I[i,c]                 (d*Dr square)
+mk_status k_asm_cp(const double* C, const double* D, const double* I, int64_t Dr, int d, double* out,
+                   hipStream_t st);
+
 }  // namespace mk

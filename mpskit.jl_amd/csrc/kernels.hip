@@ -529,6 +529,97 @@ __global__ __launch_bounds__(256) void nn_prologue_kernel(const double* __restri
                           (int64_t)blockIdx.x * PRO_A, (int64_t)blockIdx.y * PRO_C, t);
 }
 
+
+// ---- Jordan operator matrices from constructed terms (prepare_operator!! folds)
+// Absent terms are null pointers; flags say which optional terms a target takes.
+
+__global__ void asm_ic_kernel(const double* __restrict__ IC, const double* __restrict__ ID,
+                              const double* __restrict__ II, int64_t Dr, int d2, double* __restrict__ out) {
+  const int64_t N = Dr * d2;
+  GRID_STRIDE(idx, N * N) {
+    const int64_t row = idx % N, col = idx / N;
+    const int64_t i = row % Dr, c = col % Dr;
+    const int j = (int)(row / Dr), e = (int)(col / Dr);
+    double v = IC ? IC[e + d2 * (c + Dr * (j + d2 * i))] : 0.0;
+    if (ID && c == i) v += ID[e + d2 * j];
+    if (II && e == j) v += II[c + Dr * i];
+    out[idx] = v;
+  }
+}
+
+__global__ void asm_be_kernel(const double* __restrict__ BE, const double* __restrict__ DE,
+                              const double* __restrict__ EE, int64_t Dl, int d1, double* __restrict__ out) {
+  const int64_t M = Dl * d1;
+  GRID_STRIDE(idx, M * M) {
+    const int64_t row = idx % M, col = idx / M;
+    const int64_t a = row % Dl, f = col % Dl;
+    const int b = (int)(row / Dl), h = (int)(col / Dl);
+    double v = BE ? BE[idx] : 0.0;
+    if (DE && a == f) v += DE[b + d1 * h];
+    if (EE && b == h) v += EE[a + Dl * f];
+    out[idx] = v;
+  }
+}
+
+__global__ void asm_ca_kernel(const double* __restrict__ CA, const double* __restrict__ CB,
+                              const double* __restrict__ IC, const double* __restrict__ ID,
+                              const double* __restrict__ II, int64_t Dr, int d1, int d2, double* __restrict__ out) {
+  const int64_t N = Dr * d2, R = d1 * N;
+  GRID_STRIDE(idx, R * R) {
+    int64_t t = idx % R;
+    const int h = (int)(t % d1); t /= d1;
+    const int64_t i = t % Dr;
+    const int j = (int)(t / Dr);
+    t = idx / R;
+    const int b = (int)(t % d1); t /= d1;
+    const int64_t c = t % Dr;
+    const int e = (int)(t / Dr);
+    double v = CA[b + d1 * (e + d2 * (c + Dr * (h + d1 * (j + d2 * i))))];
+    if (CB && c == i) v += CB[b + d1 * (e + d2 * (h + d1 * j))];
+    if (b == h) {
+      if (IC) v += IC[e + d2 * (c + Dr * (j + d2 * i))];
+      if (ID && c == i) v += ID[e + d2 * j];
+      if (II && e == j) v += II[c + Dr * i];
+    }
+    out[idx] = v;
+  }
+}
+
+__global__ void asm_ab_kernel(const double* __restrict__ AB, const double* __restrict__ CB,
+                              const double* __restrict__ BE, const double* __restrict__ DE,
+                              const double* __restrict__ EE, int64_t Dl, int d1, int d2, double* __restrict__ out) {
+  const int64_t M = Dl * d1, MM = M * M;
+  GRID_STRIDE(idx, MM * d2 * d2) {
+    const int64_t blk = idx / MM, w = idx % MM;
+    const int e = (int)(blk % d2), j = (int)(blk / d2);
+    const int64_t row = w % M, col = w / M;
+    const int64_t a = row % Dl, f = col % Dl;
+    const int b = (int)(row / Dl), h = (int)(col / Dl);
+    double v = AB[a + Dl * (b + d1 * (e + d2 * (f + Dl * (h + d1 * (int64_t)j))))];
+    if (CB && a == f) v += CB[b + d1 * (e + d2 * (h + d1 * j))];
+    if (e == j) {
+      if (BE) v += BE[w];
+      if (DE && a == f) v += DE[b + d1 * h];
+      if (EE && b == h) v += EE[a + Dl * f];
+    }
+    out[idx] = v;
+  }
+}
+
+__global__ void asm_cp_kernel(const double* __restrict__ C, const double* __restrict__ D,
+                              const double* __restrict__ I, int64_t Dr, int d, double* __restrict__ out) {
+  const int64_t N = Dr * d;
+  GRID_STRIDE(idx, N * N) {
+    const int64_t row = idx % N, col = idx / N;
+    const int h = (int)(row % d), b = (int)(col % d);
+    const int64_t i = row / d, c = col / d;
+    double v = C ? C[b + d * (c + Dr * (h + d * i))] : 0.0;
+    if (D && i == c) v += D[b + d * h];
+    if (I && b == h) v += I[i + Dr * c];
+    out[idx] = v;
+  }
+}
+
 }  // namespace
 
 #define LAUNCH_1D(kern, n, st, ...)                                          \
@@ -701,6 +792,30 @@ mk_status k_nn_prologue(const double* P, const double* x, int64_t Dl, int64_t Dr
                        ka, xb_hi, nullptr, kb);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
+}
+
+
+mk_status k_asm_ic(const double* IC, const double* ID, const double* II, int64_t Dr, int d2, double* out,
+                   hipStream_t st) {
+  LAUNCH_1D(asm_ic_kernel, Dr * d2 * Dr * d2, st, IC, ID, II, Dr, d2, out);
+}
+mk_status k_asm_be(const double* BE, const double* DE, const double* EE, int64_t Dl, int d1, double* out,
+                   hipStream_t st) {
+  LAUNCH_1D(asm_be_kernel, Dl * d1 * Dl * d1, st, BE, DE, EE, Dl, d1, out);
+}
+mk_status k_asm_ca(const double* CA, const double* CB, const double* IC, const double* ID, const double* II,
+                   int64_t Dr, int d1, int d2, double* out, hipStream_t st) {
+  const int64_t R = (int64_t)d1 * Dr * d2;
+  LAUNCH_1D(asm_ca_kernel, R * R, st, CA, CB, IC, ID, II, Dr, d1, d2, out);
+}
+mk_status k_asm_ab(const double* AB, const double* CB, const double* BE, const double* DE, const double* EE,
+                   int64_t Dl, int d1, int d2, double* out, hipStream_t st) {
+  const int64_t M = Dl * d1;
+  LAUNCH_1D(asm_ab_kernel, M * M * d2 * d2, st, AB, CB, BE, DE, EE, Dl, d1, d2, out);
+}
+mk_status k_asm_cp(const double* C, const double* D, const double* I, int64_t Dr, int d, double* out,
+                   hipStream_t st) {
+  LAUNCH_1D(asm_cp_kernel, Dr * d * Dr * d, st, C, D, I, Dr, d, out);
 }
 
 }  // namespace mk

@@ -312,8 +312,8 @@ mk_status Mlp::init(uint64_t seed, hipStream_t st) {
 
 mk_status Mlp::ensure_act(int64_t rows) {
   if (rows <= act_rows) return MK_OK;
-  // earlier launches may still use the old buffers
-  if (act_rows > 0) MK_CHECK_HIP(hipDeviceSynchronize());
+  // earlier launches (all on the context's stream) may still use the old buffers
+  if (act_rows > 0) MK_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   const size_t per = (size_t)rows * 4;
   MK_TRY(act1.alloc(per * MLP_HID));
   MK_TRY(act2.alloc(per * MLP_HID));

@@ -379,13 +379,13 @@ mk_status apply_ac_jordan(Op* op, const double* x, double* y, double alpha, doub
   MixedView xc, xb;
   if (prec != MK_FP64) {
     void *mc = nullptr, *mb = nullptr;
-    MK_TRY(carve(ctx, {{view_bytes(Dl, d * Dr, 'A', prec), &mc},
+    MK_TRY(carve(ctx, {{op->Cp.present() ? view_bytes(Dl, d * Dr, 'A', prec) : 0, &mc},
                        {op->Bp.present() ? view_bytes(Dl * d, Dr, 'B', prec) : 0, &mb}}));
-    MK_TRY(convert_view(ctx, x, Dl, d * Dr, Dl, 'A', prec, &xc, mc));
+    if (op->Cp.present()) MK_TRY(convert_view(ctx, x, Dl, d * Dr, Dl, 'A', prec, &xc, mc));
     if (op->Bp.present()) MK_TRY(convert_view(ctx, x, Dl * d, Dr, Dl * d, 'B', prec, &xb, mb));
   }
   const GemmShape nn{};
-  {
+  if (op->Cp.present()) {
     // y[a,(b,c)] = x[a,(h,i)] C'[(h,i),(b,c)]
     ProbSpec p;
     p.M = Dl; p.N = d * Dr; p.C = y; p.ldc = Dl; p.alpha = alpha; p.beta = bs.next();
@@ -404,6 +404,7 @@ mk_status apply_ac_jordan(Op* op, const double* x, double* y, double alpha, doub
     MK_TRY(run_gemm(&p, 1, prec, nn, ctx->stream));
   }
   if (op->gen.present()) MK_TRY(apply_lxr(op, op->gen, x, y, alpha, bs.next(), prec));
+  if (bs.first) MK_TRY(k_axpby(0.0, y, beta, y, op->y_elems, ctx->stream));
   return MK_OK;
 }
 
@@ -779,6 +780,132 @@ extern "C" mk_status mk_prepare_ac2_jordan(mk_ctx h, const double* GL, const dou
   }
   op->x_elems = M * N;
   op->y_elems = M * N;
+  op->flops = fl;
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_prepare_ac2_jordan_terms(mk_ctx h, const mk_jordan_ac2_terms* t, int64_t Dl, int64_t d1,
+                                                 int64_t Dr, int64_t d2, int precisions, mk_op* out) {
+  MK_REQUIRE(t != nullptr, MK_BAD_ARG, "prepare_ac2_jordan_terms: null terms");
+  MK_REQUIRE(okdim(Dl) && okdim(d1) && okdim(Dr) && okdim(d2), MK_BAD_SHAPE,
+             "prepare_ac2_jordan_terms: non-positive dimension");
+  MK_REQUIRE(d2 <= MAX_PROB && d2 <= MAX_SEG, MK_NOT_SUPPORTED, "prepare_ac2_jordan_terms: d2 > %d", MAX_SEG);
+  const bool hasAA = t->AA_GL || t->AA_W1 || t->AA_W2 || t->AA_GR;
+  MK_REQUIRE(!hasAA || (t->AA_GL && t->AA_W1 && t->AA_W2 && t->AA_GR && okdim(t->aa_wl) && okdim(t->aa_wm) &&
+                        okdim(t->aa_wr)),
+             MK_BAD_ARG, "prepare_ac2_jordan_terms: AA needs GL, W1, W2, GR and positive channel counts");
+  MK_REQUIRE(t->II || t->IC || t->ID || t->CB || t->CA || t->AB || t->BE || t->DE || t->EE || hasAA, MK_BAD_ARG,
+             "prepare_ac2_jordan_terms: every term is missing");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_AC2_JOR, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  op->Dl = Dl; op->Dr = Dr; op->d1 = d1; op->d2 = d2;
+  const int64_t M = Dl * d1, N = Dr * d2;
+  const bool hasCA = t->CA != nullptr, hasAB = t->AB != nullptr;
+  double fl = 0.0;
+  // prepare_operator!! folds (hamiltonian_derivatives.jl:379-462): CB into CA,
+  // else into AB; IC, ID, II into CA when it exists; BE, DE, EE into AB when it
+  // exists.  Terms the reference would keep apart (II / ID with neither IC nor
+  // CA, DE / EE with neither BE nor AB) are added into an IC / BE of their own:
+  // the same operator.
+  if (hasCA) {
+    OP_TRY(op->CA.alloc('B', d1 * N, d1 * N));
+    OP_TRY(k_asm_ca(t->CA, t->CB, t->IC, t->ID, t->II, Dr, (int)d1, (int)d2, op->CA.f64(), st));
+    fl += 2.0 * Dl * (double)(d1 * N) * (d1 * N);
+  } else if (t->IC || t->ID || t->II) {
+    OP_TRY(op->IC.alloc('B', N, N));
+    OP_TRY(k_asm_ic(t->IC, t->ID, t->II, Dr, (int)d2, op->IC.f64(), st));
+    fl += 2.0 * M * (double)N * N;
+  }
+  if (hasAB) {
+    OP_TRY(op->ABall.alloc((size_t)M * M * d2 * d2 * 8));
+    OP_TRY(k_asm_ab(t->AB, hasCA ? nullptr : t->CB, t->BE, t->DE, t->EE, Dl, (int)d1, (int)d2,
+                    static_cast<double*>(op->ABall.ptr), st));
+    op->AB.resize(d2 * d2);
+    for (int64_t b = 0; b < d2 * d2; ++b)
+      op->AB[b].view('A', static_cast<double*>(op->ABall.ptr) + b * M * M, M, M);
+    fl += 2.0 * d2 * d2 * (double)M * M * Dr;
+  } else if (t->BE || t->DE || t->EE) {
+    OP_TRY(op->BE.alloc('A', M, M));
+    OP_TRY(k_asm_be(t->BE, t->DE, t->EE, Dl, (int)d1, op->BE.f64(), st));
+    fl += 2.0 * M * (double)M * N;
+  }
+  if (t->CB && !hasCA && !hasAB) {
+    const size_t nb = (size_t)d1 * d2 * d1 * d2 * 8;
+    OP_TRY(op->P.alloc(nb));
+    OP_TRY(hipMemcpyAsync(op->P.ptr, t->CB, nb, hipMemcpyDeviceToDevice, st) == hipSuccess ? MK_OK : MK_HIP_ERROR);
+    op->hasP = true;
+    fl += 2.0 * d1 * d2 * (double)d1 * d2 * Dl * Dr;
+  }
+  if (hasAA) {
+    // AA is the reference's MPO_AC2_Hamiltonian over the connected channels
+    Wh A, B;
+    A.wl = (int)t->aa_wl; A.d = (int)d1; A.wr = (int)t->aa_wm;
+    B.wl = (int)t->aa_wm; B.d = (int)d2; B.wr = (int)t->aa_wr;
+    OP_TRY(fetch(t->AA_W1, (size_t)A.wl * d1 * d1 * A.wr, A.v));
+    OP_TRY(fetch(t->AA_W2, (size_t)B.wl * d2 * d2 * B.wr, B.v));
+    std::vector<int> kl(A.wr);
+    for (int k = 0; k < A.wr; ++k) kl[k] = k;
+    OP_TRY(build_lxr(op, op->gen, t->AA_GL, A, Dl, Dl, -1, t->AA_GR, B, Dr, Dr, -1, kl));
+    fl += op->gen.flops();
+  }
+  op->x_elems = M * N;
+  op->y_elems = M * N;
+  op->flops = fl;
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_prepare_ac_jordan_terms(mk_ctx h, const mk_jordan_ac_terms* t, int64_t Dl, int64_t d,
+                                                int64_t Dr, int precisions, mk_op* out) {
+  MK_REQUIRE(t != nullptr, MK_BAD_ARG, "prepare_ac_jordan_terms: null terms");
+  MK_REQUIRE(okdim(Dl) && okdim(d) && okdim(Dr), MK_BAD_SHAPE, "prepare_ac_jordan_terms: non-positive dimension");
+  const bool hasA = t->A_GL || t->A_W || t->A_GR;
+  MK_REQUIRE(!hasA || (t->A_GL && t->A_W && t->A_GR && okdim(t->a_wl) && okdim(t->a_wr)), MK_BAD_ARG,
+             "prepare_ac_jordan_terms: A needs GL, W, GR and positive channel counts");
+  MK_REQUIRE(t->D || t->I || t->E || t->C || t->B || hasA, MK_BAD_ARG, "prepare_ac_jordan_terms: every term is missing");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_AC_JOR, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  op->Dl = Dl; op->Dr = Dr; op->d1 = d; op->d2 = 1;
+  double fl = 0.0;
+  // prepare_operator!! folds (hamiltonian_derivatives.jl:109-157): D into C,
+  // else into B; I into C; E into B.  A D or I with no C goes into a C' of its
+  // own, an E with no B into a B' of its own (the same operator).
+  const bool d_to_c = t->C || !t->B;
+  if (t->C || t->I || (t->D && d_to_c)) {
+    OP_TRY(op->Cp.alloc('B', d * Dr, d * Dr));
+    OP_TRY(k_asm_cp(t->C, d_to_c ? t->D : nullptr, t->I, Dr, (int)d, op->Cp.f64(), st));
+    fl += 2.0 * Dl * (double)(d * Dr) * (d * Dr);
+  }
+  if (t->B || t->E || (t->D && !d_to_c)) {
+    OP_TRY(op->Bp.alloc('A', Dl * d, Dl * d));
+    OP_TRY(k_asm_be(t->B, d_to_c ? nullptr : t->D, t->E, Dl, (int)d, op->Bp.f64(), st));
+    fl += 2.0 * (double)(Dl * d) * (Dl * d) * Dr;
+  }
+  if (hasA) {
+    // A is the reference's MPO_AC_Hamiltonian over the middle channels
+    LxR& gb = op->gen;
+    Wh Wa;
+    Wa.wl = (int)t->a_wl; Wa.d = (int)d; Wa.wr = (int)t->a_wr;
+    OP_TRY(fetch(t->A_W, (size_t)Wa.wl * d * d * Wa.wr, Wa.v));
+    std::vector<int> kl(Wa.wr);
+    for (int k = 0; k < Wa.wr; ++k) kl[k] = k;
+    DevBuf wad, kd;
+    OP_TRY(upload(Wa.v.data(), Wa.v.size() * 8, wad, st));
+    OP_TRY(upload(kl.data(), kl.size() * sizeof(int), kd, st));
+    const int nA = Wa.wr;
+    OP_TRY(gb.L.alloc('A', Dl * d, Dl * d * nA));
+    OP_TRY(k_prep_L(t->A_GL, static_cast<double*>(wad.ptr), static_cast<int*>(kd.ptr), nA, Dl, Dl, Wa.wl, (int)d,
+                    Wa.wr, -1, gb.L.f64(), st));
+    OP_TRY(gb.R.alloc('B', Dr, nA * Dr));
+    OP_TRY(hipMemcpyAsync(gb.R.f64(), t->A_GR, (size_t)Dr * nA * Dr * 8, hipMemcpyDeviceToDevice, st) == hipSuccess
+               ? MK_OK : MK_HIP_ERROR);
+    OP_TRY(hipStreamSynchronize(st) == hipSuccess ? MK_OK : MK_HIP_ERROR);
+    gb.Mx = Dl * d; gb.Ki = Dr; gb.nk = nA; gb.Ny = Dr; gb.Mo = Dl * d;
+    fl += gb.flops();
+  }
+  op->x_elems = Dl * d * Dr;
+  op->y_elems = Dl * d * Dr;
   op->flops = fl;
   return finish(op, out);
 }

@@ -9,6 +9,7 @@ from . import models, tensors
 from ._lib import BF16X3, FP32, FP64, EXPORTED, LIB_PATH, MKError, NotConvergedError, SpaceMismatch, load
 from .derivatives import (
     BatchedDerivative, autotune_stats, DerivativeOperator, JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, MPO_AC2_Hamiltonian, gemm,
+    JordanMPO_AC2_Terms, JordanMPO_AC_Terms,
     MPO_AC_Hamiltonian, MPO_C_Hamiltonian, MPODerivativeOperator, PreparedDerivative, prepare_operator,
 )
 from .transfer import (
@@ -21,6 +22,7 @@ __all__ = [
     "SpaceMismatch", "load", "gemm", "BatchedDerivative", "autotune_stats", "DerivativeOperator", "JordanMPO_AC2_Hamiltonian", "JordanMPO_AC_Hamiltonian",
     "MPO_AC2_Hamiltonian", "MPO_AC_Hamiltonian", "MPO_C_Hamiltonian", "MPODerivativeOperator",
     "PreparedDerivative", "prepare_operator", "ProductTransferMatrix", "RegTransferMatrix",
+    "JordanMPO_AC2_Terms", "JordanMPO_AC_Terms",
     "SingleTransferMatrix", "TransferMatrix", "regularize", "regularize_", "transfer_left", "transfer_right",
 ]
 from .algorithms import (

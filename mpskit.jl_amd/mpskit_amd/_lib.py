@@ -43,6 +43,20 @@ _P = C.c_void_p
 _D = C.c_double
 _INT = C.c_int
 
+
+class JordanAC2Terms(C.Structure):
+    """``mk_jordan_ac2_terms``: device pointers of the constructed two-site Jordan
+    terms (NULL = missing) and the AA operator's channel counts."""
+    _fields_ = [(n, _P) for n in ("II", "IC", "ID", "CB", "CA", "AB", "BE", "DE", "EE",
+                                  "AA_GL", "AA_W1", "AA_W2", "AA_GR")] + \
+               [("aa_wl", _I64), ("aa_wm", _I64), ("aa_wr", _I64)]
+
+
+class JordanACTerms(C.Structure):
+    """``mk_jordan_ac_terms``: the constructed one-site Jordan terms."""
+    _fields_ = [(n, _P) for n in ("D", "I", "E", "C", "B", "A_GL", "A_W", "A_GR")] + \
+               [("a_wl", _I64), ("a_wr", _I64)]
+
 _SIGS = {
     "mk_version": ([], C.c_int),
     "mk_last_error": ([], C.c_char_p),
@@ -55,6 +69,9 @@ _SIGS = {
     "mk_prepare_ac2_generic": ([_P, _P, _P, _P, _P] + [_I64] * 9 + [_INT, C.POINTER(_P)], _INT),
     "mk_prepare_ac_jordan": ([_P, _P, _P, _P] + [_I64] * 5 + [_INT, C.POINTER(_P)], _INT),
     "mk_prepare_ac2_jordan": ([_P, _P, _P, _P, _P] + [_I64] * 7 + [_INT, C.POINTER(_P)], _INT),
+    "mk_prepare_ac2_jordan_terms": ([_P, C.POINTER(JordanAC2Terms), _I64, _I64, _I64, _I64, _INT, C.POINTER(_P)],
+                                    _INT),
+    "mk_prepare_ac_jordan_terms": ([_P, C.POINTER(JordanACTerms), _I64, _I64, _I64, _INT, C.POINTER(_P)], _INT),
     "mk_op_set_charge_blocks": ([_P, _INT, C.POINTER(_I64), C.POINTER(_INT), _INT, C.POINTER(_I64), C.POINTER(_INT),
                                  C.POINTER(_INT)], _INT),
     "mk_apply": ([_P, _P, _P, _D, _D, _INT], _INT),

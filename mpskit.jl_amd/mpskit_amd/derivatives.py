@@ -10,6 +10,9 @@ Mirrors MPSKit's operator family (src/algorithms/derivatives/):
   JordanMPO_AC_Hamiltonian     hamiltonian_derivatives.jl:15-107
   JordanMPO_AC2_Hamiltonian    hamiltonian_derivatives.jl:168-369
   prepare_operator             prepare_operator!! (derivatives.jl:236-246)
+  JordanMPO_AC2_Terms, JordanMPO_AC_Terms
+                               constructed (unprepared) Jordan operators,
+                               prepared by the one-argument prepare_operator!!
 
 Every operator is prepared once into an immutable ``mk_op`` handle; applying
 it enqueues the HIP kernels on torch's current stream.  Inputs/outputs are
@@ -272,6 +275,102 @@ class JordanMPO_AC2_Hamiltonian(DerivativeOperator):
         return self._prepared(x, precision, **kw)
 
 
+class JordanMPO_AC2_Terms(DerivativeOperator):
+    """A constructed, not yet prepared ``JordanMPO_AC2_Hamiltonian``: its fields
+    (hamiltonian_derivatives.jl:168-200) as column-major device tensors, None
+    for ``missing``.  Layouts: II[c,i], IC[e,c,j,i], ID[e,j], CB[b,e,h,j],
+    CA[b,e,c,h,j,i], AB[a,b,e,f,h,j], BE[a,b,f,h], DE[b,h], EE[a,f]; AA is the
+    tuple (GL, W1, W2, GR) of the generic operator over the connected channels.
+    ``dims`` = (Dl, d1, Dr, d2).  ``prepare_operator`` folds the terms as the
+    reference's one-argument ``prepare_operator!!`` does (:371-468)."""
+
+    KEYS = ("II", "IC", "ID", "CB", "CA", "AB", "BE", "DE", "EE")
+
+    def __init__(self, dims, AA=None, **terms):
+        unknown = set(terms) - set(self.KEYS)
+        if unknown:
+            raise TypeError(f"JordanMPO_AC2_Terms: unknown terms {sorted(unknown)}")
+        self.dims = tuple(int(n) for n in dims)
+        self.terms = {k: terms.get(k) for k in self.KEYS}
+        self.AA = AA
+        self._prepared = None
+
+    def __call__(self, x, precision: PrecisionArg = "fp64", **kw):
+        if self._prepared is None:
+            self._prepared = prepare_operator(self)
+        return self._prepared(x, precision, **kw)
+
+
+class JordanMPO_AC_Terms(DerivativeOperator):
+    """A constructed, not yet prepared ``JordanMPO_AC_Hamiltonian`` (fields at
+    hamiltonian_derivatives.jl:15-32): D[b,h], I[i,c], E[a,f], C[b,c,h,i],
+    B[a,b,f,h], and A = (GL, W, GR) of the generic operator over the middle
+    channels.  ``dims`` = (Dl, d, Dr)."""
+
+    KEYS = ("D", "I", "E", "C", "B")
+
+    def __init__(self, dims, A=None, **terms):
+        unknown = set(terms) - set(self.KEYS)
+        if unknown:
+            raise TypeError(f"JordanMPO_AC_Terms: unknown terms {sorted(unknown)}")
+        self.dims = tuple(int(n) for n in dims)
+        self.terms = {k: terms.get(k) for k in self.KEYS}
+        self.A = A
+        self._prepared = None
+
+    def __call__(self, x, precision: PrecisionArg = "fp64", **kw):
+        if self._prepared is None:
+            self._prepared = prepare_operator(self)
+        return self._prepared(x, precision, **kw)
+
+
+def _prepare_terms(H, ctx, mask) -> PreparedDerivative:
+    from ._lib import JordanAC2Terms, JordanACTerms
+    lib = load()
+    h = C.c_void_p()
+    keep = []                                  # column-major copies stay alive during the call
+
+    def ptr(t, shape, name):
+        if t is None:
+            return None
+        t = _req(t, shape, name)
+        keep.append(t)
+        return T.ptr(t)
+
+    if isinstance(H, JordanMPO_AC2_Terms):
+        Dl, d1, Dr, d2 = H.dims
+        shapes = dict(II=(Dr, Dr), IC=(d2, Dr, d2, Dr), ID=(d2, d2), CB=(d1, d2, d1, d2),
+                      CA=(d1, d2, Dr, d1, d2, Dr), AB=(Dl, d1, d2, Dl, d1, d2), BE=(Dl, d1, Dl, d1), DE=(d1, d1),
+                      EE=(Dl, Dl))
+        s = JordanAC2Terms()
+        for k in H.KEYS:
+            setattr(s, k, ptr(H.terms[k], shapes[k], k))
+        if H.AA is not None:
+            GL, W1, W2, GR = H.AA
+            wl, wm, wr = GL.shape[1], W1.shape[3], GR.shape[1]
+            s.AA_GL = ptr(GL, (Dl, wl, Dl), "AA GL")
+            s.AA_W1 = ptr(W1, (wl, d1, d1, wm), "AA W1")
+            s.AA_W2 = ptr(W2, (wm, d2, d2, wr), "AA W2")
+            s.AA_GR = ptr(GR, (Dr, wr, Dr), "AA GR")
+            s.aa_wl, s.aa_wm, s.aa_wr = wl, wm, wr
+        check(lib.mk_prepare_ac2_jordan_terms(ctx.handle, C.byref(s), Dl, d1, Dr, d2, mask, C.byref(h)))
+        return PreparedDerivative(h, ctx, (Dl, d1, Dr, d2), (Dl, d1, Dr, d2), "JordanAC2", mask)
+    Dl, d, Dr = H.dims
+    shapes = dict(D=(d, d), I=(Dr, Dr), E=(Dl, Dl), C=(d, Dr, d, Dr), B=(Dl, d, Dl, d))
+    s = JordanACTerms()
+    for k in H.KEYS:
+        setattr(s, k, ptr(H.terms[k], shapes[k], k))
+    if H.A is not None:
+        GL, W, GR = H.A
+        wl, wr = GL.shape[1], GR.shape[1]
+        s.A_GL = ptr(GL, (Dl, wl, Dl), "A GL")
+        s.A_W = ptr(W, (wl, d, d, wr), "A W")
+        s.A_GR = ptr(GR, (Dr, wr, Dr), "A GR")
+        s.a_wl, s.a_wr = wl, wr
+    check(lib.mk_prepare_ac_jordan_terms(ctx.handle, C.byref(s), Dl, d, Dr, mask, C.byref(h)))
+    return PreparedDerivative(h, ctx, (Dl, d, Dr), (Dl, d, Dr), "JordanAC", mask)
+
+
 def prepare_operator(H, precisions=None) -> PreparedDerivative:
     """``prepare_operator!!`` (derivatives.jl:236-246, mpo_derivatives.jl:216-247,
     hamiltonian_derivatives.jl:109-157, :371-468)."""
@@ -281,6 +380,8 @@ def prepare_operator(H, precisions=None) -> PreparedDerivative:
     ctx = default_context()
     mask = _mask(precisions)
     h = C.c_void_p()
+    if isinstance(H, (JordanMPO_AC2_Terms, JordanMPO_AC_Terms)):
+        return _prepare_terms(H, ctx, mask)
     if isinstance(H, MPODerivativeOperator):
         GL, GR = H.leftenv, H.rightenv
         if GL.dim() != 3 or GR.dim() != 3:

@@ -42,8 +42,8 @@ __all__ = [
     "F", "fmat", "ac2_apply_generic", "ac_apply_generic", "c_apply",
     "prepare_ac2_generic", "apply_prepared_ac2", "prepare_ac_generic",
     "apply_prepared_ac", "prepare_c", "apply_prepared_c",
-    "jordan_split", "jordan_ac2_prepare", "jordan_ac2_apply",
-    "jordan_ac_prepare", "jordan_ac_apply",
+    "jordan_split", "jordan_ac2_construct", "jordan_ac2_fold", "jordan_ac2_prepare", "jordan_ac2_apply",
+    "jordan_ac_construct", "jordan_ac_fold", "jordan_ac_prepare", "jordan_ac_apply",
     "transfer_left_mps", "transfer_right_mps", "transfer_left_mpo",
     "transfer_right_mpo", "regularize",
 ]
@@ -200,18 +200,16 @@ def _connected_channels(A1, A2):
     return rows, mids, cols
 
 
-def jordan_ac2_prepare(GL, W1, W2, GR):
-    """``JordanMPO_AC2_Hamiltonian`` constructor + ``prepare_operator!!``
-    (hamiltonian_derivatives.jl:272-369 and :371-468).  Returns a dict of the
-    surviving terms (None == ``missing``) with the reference's tensor layouts:
+def jordan_ac2_construct(GL, W1, W2, GR):
+    """``JordanMPO_AC2_Hamiltonian(GL, W1, W2, GR)`` constructor
+    (hamiltonian_derivatives.jl:272-369): the unprepared terms (None ==
+    ``missing``) in the reference's tensor layouts:
       II[c,i], IC[e,c,j,i], ID[e,j], CB[b,e,h,j], CA[b,e,c,h,j,i],
       AB[a,b,e,f,h,j], BE[a,b,f,h], DE[b,h], EE[a,f], AA=(GL2,A1,A2,GR2) or None.
     """
     b1 = jordan_split(W1)
     b2 = jordan_split(W2)
-    Dl, Dr = GL.shape[0], GR.shape[0]
     d1, d2 = W1.shape[1], W2.shape[1]
-    mid = b1["mid_r"]                                  # == b2["mid_l"]
     GL2 = GL[:, b1["mid_l"], :] if b1["mid_l"] else None
     GR2 = GR[:, b2["mid_r"], :] if b2["mid_r"] else None
 
@@ -233,34 +231,47 @@ def jordan_ac2_prepare(GL, W1, W2, GR):
         rows, mids, cols = ch
         AA = (GL2[:, rows, :], A1[np.ix_(rows, range(d1), range(d1), mids)],
               A2[np.ix_(mids, range(d2), range(d2), cols)], GR2[:, cols, :])
-
-    eye = np.eye
-    # prepare_operator!! folding, :379-462 (same order, same fallbacks)
-    if CA is not None and CB is not None:
-        CA = CA + _es("behj,ci->bechji", CB, eye(Dr)); CB = None
-    elif AB is not None and CB is not None:
-        AB = AB + _es("behj,af->abefhj", CB, eye(Dl)); CB = None
-    if CA is not None and IC is not None:
-        CA = CA + _es("bh,ecji->bechji", eye(d1), IC); IC = None
-    if AB is not None and BE is not None:
-        AB = AB + _es("abfh,ej->abefhj", BE, eye(d2)); BE = None
-    if BE is not None and DE is not None:
-        BE = BE + _es("af,bh->abfh", eye(Dl), DE); DE = None
-    elif AB is not None and DE is not None:
-        AB = AB + _es("af,bh,ej->abefhj", eye(Dl), DE, eye(d2)); DE = None
-    if IC is not None and ID is not None:
-        IC = IC + _es("ej,ci->ecji", ID, eye(Dr)); ID = None
-    elif CA is not None and ID is not None:
-        CA = CA + _es("bh,ej,ci->bechji", eye(d1), ID, eye(Dr)); ID = None
-    if IC is not None and II is not None:
-        IC = IC + _es("ej,ci->ecji", eye(d2), II); II = None
-    elif CA is not None and II is not None:
-        CA = CA + _es("bh,ej,ci->bechji", eye(d1), eye(d2), II); II = None
-    if BE is not None and EE is not None:
-        BE = BE + _es("af,bh->abfh", EE, eye(d1)); EE = None
-    elif AB is not None and EE is not None:
-        AB = AB + _es("af,bh,ej->abefhj", EE, eye(d1), eye(d2)); EE = None
     return dict(II=II, IC=IC, ID=ID, CB=CB, CA=CA, AB=AB, AA=AA, BE=BE, DE=DE, EE=EE)
+
+
+def jordan_ac2_fold(T):
+    """``prepare_operator!!(::JordanMPO_AC2_Hamiltonian)`` (hamiltonian_derivatives.jl:371-468):
+    folds the small terms into the large ones with ``+= Id (x) X``, in the
+    reference's order and with its fallbacks.  Returns a new dict."""
+    II, IC, ID, CB, CA, AB, AA, BE, DE, EE = (T[k] for k in ("II", "IC", "ID", "CB", "CA", "AB", "AA", "BE",
+                                                            "DE", "EE"))
+    eye = np.eye
+    if CA is not None and CB is not None:                           # :379-389
+        CA = CA + _es("behj,ci->bechji", CB, eye(CA.shape[2])); CB = None
+    elif AB is not None and CB is not None:
+        AB = AB + _es("behj,af->abefhj", CB, eye(AB.shape[0])); CB = None
+    if CA is not None and IC is not None:                           # :391-398
+        CA = CA + _es("bh,ecji->bechji", eye(CA.shape[0]), IC); IC = None
+    if AB is not None and BE is not None:                           # :400-407
+        AB = AB + _es("abfh,ej->abefhj", BE, eye(AB.shape[2])); BE = None
+    if BE is not None and DE is not None:                           # :409-421
+        BE = BE + _es("af,bh->abfh", eye(BE.shape[0]), DE); DE = None
+    elif AB is not None and DE is not None:
+        AB = AB + _es("af,bh,ej->abefhj", eye(AB.shape[0]), DE, eye(AB.shape[2])); DE = None
+    if IC is not None and ID is not None:                           # :423-434
+        IC = IC + _es("ej,ci->ecji", ID, eye(IC.shape[1])); ID = None
+    elif CA is not None and ID is not None:
+        CA = CA + _es("bh,ej,ci->bechji", eye(CA.shape[0]), ID, eye(CA.shape[2])); ID = None
+    if IC is not None and II is not None:                           # :436-446
+        IC = IC + _es("ej,ci->ecji", eye(IC.shape[0]), II); II = None
+    elif CA is not None and II is not None:
+        CA = CA + _es("bh,ej,ci->bechji", eye(CA.shape[0]), eye(CA.shape[1]), II); II = None
+    if BE is not None and EE is not None:                           # :448-458
+        BE = BE + _es("af,bh->abfh", EE, eye(BE.shape[1])); EE = None
+    elif AB is not None and EE is not None:
+        AB = AB + _es("af,bh,ej->abefhj", EE, eye(AB.shape[1]), eye(AB.shape[2])); EE = None
+    return dict(II=II, IC=IC, ID=ID, CB=CB, CA=CA, AB=AB, AA=AA, BE=BE, DE=DE, EE=EE)
+
+
+def jordan_ac2_prepare(GL, W1, W2, GR):
+    """``AC2_hamiltonian(...; prepare = true)`` (hamiltonian_derivatives.jl:206-217):
+    the constructor followed by ``prepare_operator!!``."""
+    return jordan_ac2_fold(jordan_ac2_construct(GL, W1, W2, GR))
 
 
 def jordan_ac2_apply(T, x):
@@ -278,30 +289,42 @@ def jordan_ac2_apply(T, x):
     return y
 
 
-def jordan_ac_prepare(GL, W, GR):
-    """``JordanMPO_AC_Hamiltonian`` constructor + ``prepare_operator!!``
-    (hamiltonian_derivatives.jl:57-107, :109-157).  Layouts:
-    D[b,h], I[i,c], E[a,f], C[b,c,h,i], B[a,b,f,h], A=(GL2,WA,GR2) or None."""
+def jordan_ac_construct(GL, W, GR):
+    """``JordanMPO_AC_Hamiltonian(GL, W, GR)`` constructor
+    (hamiltonian_derivatives.jl:57-107): unprepared terms D[b,h], I[i,c],
+    E[a,f], C[b,c,h,i], B[a,b,f,h], A=(GL2,WA,GR2) or None (None == missing)."""
     blk = jordan_split(W)
-    Dl, Dr, d = GL.shape[0], GR.shape[0], W.shape[1]
     GL2 = GL[:, blk["mid_l"], :] if blk["mid_l"] else None
     GR2 = GR[:, blk["mid_r"], :] if blk["mid_r"] else None
-    Dm = blk["D"].copy() if _nonzero(blk["D"]) else None
-    I = GR[:, 0, :].copy() if W.shape[3] > 1 else None
-    E = GL[:, -1, :].copy() if W.shape[0] > 1 else None
-    C = _es("bhk,ikc->bchi", blk["C"], GR2) if _nonzero(blk["C"]) else None
-    B = _es("akf,kbh->abfh", GL2, blk["B"]) if _nonzero(blk["B"]) else None
-    A = (GL2, blk["A"], GR2) if _nonzero(blk["A"]) else None
+    Dm = blk["D"].copy() if _nonzero(blk["D"]) else None                # :68
+    I = GR[:, 0, :].copy() if W.shape[3] > 1 else None                  # :71
+    E = GL[:, -1, :].copy() if W.shape[0] > 1 else None                 # :74
+    C = _es("bhk,ikc->bchi", blk["C"], GR2) if _nonzero(blk["C"]) else None    # :77-82
+    B = _es("akf,kbh->abfh", GL2, blk["B"]) if _nonzero(blk["B"]) else None    # :85-90
+    A = (GL2, blk["A"], GR2) if _nonzero(blk["A"]) else None                   # :93, :104
+    return dict(D=Dm, I=I, E=E, C=C, B=B, A=A)
+
+
+def jordan_ac_fold(T):
+    """``prepare_operator!!(::JordanMPO_AC_Hamiltonian)`` (hamiltonian_derivatives.jl:109-157):
+    D into C (else B), I into C, E into B."""
+    Dm, I, E, C, B, A = (T[k] for k in ("D", "I", "E", "C", "B", "A"))
     if Dm is not None:
         if C is not None:
-            C = C + _es("bh,ci->bchi", Dm, np.eye(Dr)); Dm = None
+            C = C + _es("bh,ci->bchi", Dm, np.eye(C.shape[1])); Dm = None
         elif B is not None:
-            B = B + _es("af,bh->abfh", np.eye(Dl), Dm); Dm = None
+            B = B + _es("af,bh->abfh", np.eye(B.shape[0]), Dm); Dm = None
     if I is not None and C is not None:
-        C = C + _es("bh,ic->bchi", np.eye(d), I); I = None
+        C = C + _es("bh,ic->bchi", np.eye(C.shape[0]), I); I = None
     if E is not None and B is not None:
-        B = B + _es("af,bh->abfh", E, np.eye(d)); E = None
+        B = B + _es("af,bh->abfh", E, np.eye(B.shape[1])); E = None
     return dict(D=Dm, I=I, E=E, C=C, B=B, A=A)
+
+
+def jordan_ac_prepare(GL, W, GR):
+    """``AC_hamiltonian(...; prepare = true)`` (hamiltonian_derivatives.jl:44-55).  Layouts:
+    D[b,h], I[i,c], E[a,f], C[b,c,h,i], B[a,b,f,h], A=(GL2,WA,GR2) or None."""
+    return jordan_ac_fold(jordan_ac_construct(GL, W, GR))
 
 
 def jordan_ac_apply(T, x):

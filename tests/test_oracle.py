@@ -194,6 +194,36 @@ def test_mps_transfers_vs_kronecker_matrices():
     assert rel(O.transfer_right_mps(np.eye(D), Ar.transpose(2, 1, 0), Ar.transpose(2, 1, 0)), np.eye(D)) > 1e-3
 
 
+@pytest.mark.parametrize("name", list(CHAINS))
+def test_jordan_prepare_does_not_change_the_operator(name):
+    """prepare_operator!! only regroups terms (hamiltonian_derivatives.jl:109-157,
+    :371-468): the constructed operator, whose apply runs every term on its own
+    (II, IC, ID, CB, CA, AB, AA, BE, DE, EE and D, I, E, C, B, A), equals the
+    folded one, which the projected-H pins above check.  Every bond of the chain,
+    boundaries included, so each unfolded term's apply is exercised."""
+    rng = np.random.default_rng(17)
+    Ws = CHAINS[name][0]()
+    psi = B.random_mps(L, 2, 6, rng)
+    (AL, AR, _, _), GL, GR = envs(Ws, psi, psi)
+    seen = set()
+    for j in range(L - 1):
+        T0 = O.jordan_ac2_construct(GL[j], Ws[j], Ws[j + 1], GR[j + 2])
+        seen |= {k for k, v in T0.items() if v is not None}
+        x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 2].shape[0], 2))
+        assert rel(O.jordan_ac2_apply(T0, x), O.jordan_ac2_apply(O.jordan_ac2_fold(T0), x)) < 1e-13
+    for j in range(L):
+        T0 = O.jordan_ac_construct(GL[j], Ws[j], GR[j + 1])
+        x = rng.standard_normal((GL[j].shape[2], 2, GR[j + 1].shape[0]))
+        assert rel(O.jordan_ac_apply(T0, x), O.jordan_ac_apply(O.jordan_ac_fold(T0), x)) < 1e-13
+    assert {"II", "IC", "CB", "BE", "EE"} <= seen
+    if name in ("j1j2", "heis3"):
+        assert {"CA", "AB"} <= seen
+    if name in ("heis3", "expising"):
+        assert "AA" in seen
+    if name == "tfim":
+        assert {"DE", "ID"} <= seen
+
+
 def test_jordan_uses_identity_corner_levels():
     """Jordan applies ignore GL's "not started" and GR's "finished" levels (taken
     as identities), so they equal the generic apply with those levels set to 1."""
