@@ -160,6 +160,55 @@ mk_status mk_prepare_ac2_jordan_terms(mk_ctx ctx, const mk_jordan_ac2_terms* ter
 mk_status mk_prepare_ac_jordan_terms(mk_ctx ctx, const mk_jordan_ac_terms* terms, int64_t Dl, int64_t d,
                                      int64_t Dr, int precisions, mk_op* out);
 
+/* ---- block-stored environments and MPO tensors -------------------------------
+ * The layouts the reference's MPOHamiltonian path holds, passed without any
+ * densification by the caller:
+ *   - an environment is a BlockTensorMap over a SumSpace on its MPO leg, one
+ *     TensorMap per Jordan level (src/environments/abstract_envs.jl:32-58):
+ *     mk_env_levels lists the device pointers of the levels' `.data`, level L a
+ *     column-major D1 x width[L] x D2 array; the levels in channel order;
+ *   - a JordanMPOTensor stores its nonzero blocks, identity blocks as scalars
+ *     (src/operators/jordanmpotensor.jl:44-69): mk_jordan_mpo lists, per block,
+ *     its channel position (row, col) and either a device pointer to the d x d
+ *     block (out x in, column-major) or NULL and a scalar (scalar x identity).
+ * The library gathers the levels on the device (one strided copy per level,
+ * stream-ordered) and assembles the MPO tensor; outputs are scattered back into
+ * the caller's levels.  These calls synchronise the context's stream before
+ * they return.  Otherwise each equals its dense counterpart, bit for bit. */
+typedef struct mk_env_levels {
+  int n;                      /* number of levels (1..64) */
+  double* const* ptr;         /* host array of n device pointers */
+  const int64_t* width;       /* host array of n channel counts */
+} mk_env_levels;
+typedef struct mk_jordan_mpo {
+  int64_t wl, d, wr;          /* channels of the left / right MPO bond, physical dimension */
+  int nblocks;
+  const int* row;             /* host arrays of nblocks entries */
+  const int* col;
+  const double* const* block; /* device pointer of the d x d block, or NULL: scalar x identity */
+  const double* scalar;
+} mk_jordan_mpo;
+/* mk_prepare_ac2_jordan / mk_prepare_ac_jordan / mk_prepare_c with block-stored
+ * environments and MPO tensors (hamiltonian_derivatives.jl:206-217, :44-55;
+ * mpo_derivatives.jl:37-44, where the reference densifies GL / GR itself). */
+mk_status mk_prepare_ac2_jordan_levels(mk_ctx ctx, const mk_env_levels* GL, const mk_jordan_mpo* W1,
+                                       const mk_jordan_mpo* W2, const mk_env_levels* GR, int64_t Dl,
+                                       int64_t Dr, int precisions, mk_op* out);
+mk_status mk_prepare_ac_jordan_levels(mk_ctx ctx, const mk_env_levels* GL, const mk_jordan_mpo* W,
+                                      const mk_env_levels* GR, int64_t Dl, int64_t Dr, int precisions,
+                                      mk_op* out);
+mk_status mk_prepare_c_levels(mk_ctx ctx, const mk_env_levels* GL, const mk_env_levels* GR, int64_t Da,
+                              int64_t Df, int64_t Di, int64_t Dc, int precisions, mk_op* out);
+/* Environment updates of the MPOHamiltonian path (transfer.jl:125-130 on
+ * BlockTensorMap environments, finite_envs.jl:110-150): x / v and the result y
+ * are block-stored; A, Abar as in mk_transfer_left_mpo / mk_transfer_right_mpo. */
+mk_status mk_transfer_left_mpo_levels(mk_ctx ctx, const mk_env_levels* x, const mk_jordan_mpo* O,
+                                      const double* A, const double* Abar, int64_t Dm, int64_t Dn,
+                                      int64_t Dc, int64_t Da, const mk_env_levels* y);
+mk_status mk_transfer_right_mpo_levels(mk_ctx ctx, const mk_env_levels* v, const mk_jordan_mpo* O,
+                                       const double* A, const double* Abar, int64_t Da, int64_t Di,
+                                       int64_t Dc, int64_t Do, const mk_env_levels* y);
+
 /* ---- U(1) charge blocks ------------------------------------------------------
  * Counterpart of TensorKit's block-sparse U(1) storage under the same
  * derivative operators (the reference applies one GEMM per fusion-tree block,

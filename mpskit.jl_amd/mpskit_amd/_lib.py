@@ -52,6 +52,17 @@ class JordanAC2Terms(C.Structure):
                [("aa_wl", _I64), ("aa_wm", _I64), ("aa_wr", _I64)]
 
 
+class EnvLevels(C.Structure):
+    """``mk_env_levels``: the levels of a block-stored environment."""
+    _fields_ = [("n", C.c_int), ("ptr", C.POINTER(_P)), ("width", C.POINTER(_I64))]
+
+
+class JordanMPO(C.Structure):
+    """``mk_jordan_mpo``: a Jordan MPO tensor as blocks (device d x d) or scalar identities."""
+    _fields_ = [("wl", _I64), ("d", _I64), ("wr", _I64), ("nblocks", C.c_int), ("row", C.POINTER(C.c_int)),
+                ("col", C.POINTER(C.c_int)), ("block", C.POINTER(_P)), ("scalar", C.POINTER(C.c_double))]
+
+
 class JordanACTerms(C.Structure):
     """``mk_jordan_ac_terms``: the constructed one-site Jordan terms."""
     _fields_ = [(n, _P) for n in ("D", "I", "E", "C", "B", "A_GL", "A_W", "A_GR")] + \
@@ -72,6 +83,16 @@ _SIGS = {
     "mk_prepare_ac2_jordan_terms": ([_P, C.POINTER(JordanAC2Terms), _I64, _I64, _I64, _I64, _INT, C.POINTER(_P)],
                                     _INT),
     "mk_prepare_ac_jordan_terms": ([_P, C.POINTER(JordanACTerms), _I64, _I64, _I64, _INT, C.POINTER(_P)], _INT),
+    "mk_prepare_ac2_jordan_levels": ([_P, C.POINTER(EnvLevels), C.POINTER(JordanMPO), C.POINTER(JordanMPO),
+                                      C.POINTER(EnvLevels), _I64, _I64, _INT, C.POINTER(_P)], _INT),
+    "mk_prepare_ac_jordan_levels": ([_P, C.POINTER(EnvLevels), C.POINTER(JordanMPO), C.POINTER(EnvLevels), _I64,
+                                     _I64, _INT, C.POINTER(_P)], _INT),
+    "mk_prepare_c_levels": ([_P, C.POINTER(EnvLevels), C.POINTER(EnvLevels)] + [_I64] * 4 + [_INT, C.POINTER(_P)],
+                            _INT),
+    "mk_transfer_left_mpo_levels": ([_P, C.POINTER(EnvLevels), C.POINTER(JordanMPO), _P, _P] + [_I64] * 4 +
+                                    [C.POINTER(EnvLevels)], _INT),
+    "mk_transfer_right_mpo_levels": ([_P, C.POINTER(EnvLevels), C.POINTER(JordanMPO), _P, _P] + [_I64] * 4 +
+                                     [C.POINTER(EnvLevels)], _INT),
     "mk_op_set_charge_blocks": ([_P, _INT, C.POINTER(_I64), C.POINTER(_INT), _INT, C.POINTER(_I64), C.POINTER(_INT),
                                  C.POINTER(_INT)], _INT),
     "mk_apply": ([_P, _P, _P, _D, _D, _INT], _INT),
