@@ -275,6 +275,25 @@ mk_status mk_transfer_right_mpo(mk_ctx ctx, const double* v, const double* O,
                                 int64_t d, int64_t Di, int64_t wl, int64_t wr,
                                 int64_t Dc, int64_t Do, double* y);
 
+/* A transfer matrix prepared for repeated applies (a TransferMatrix of the
+ * reference, src/transfermatrix/transfermatrix.jl:11-33, applied by the
+ * fixed-point solvers of src/environments/infinite_envs.jl:207-346 and
+ * src/states/ortho.jl:307-313): mk_apply(op, x, y, alpha, beta, precision)
+ * computes y = alpha T(x) + beta y in any prepared precision, mk_eigsolve and
+ * the batch solvers take it like a derivative operator when it is square.
+ * The operator keeps its own copies of O, A and Abar.  Left:
+ * T(x) = mk_transfer_left_mpo(x, O, A, Abar) with the same dimensions; right:
+ * T(v) = mk_transfer_right_mpo(v, O, A, Abar).  O = NULL gives the MPS
+ * transfer (mk_transfer_left_mps / mk_transfer_right_mps, wl = wr = 1).
+ * FP32 / BF16x3 run both GEMMs on mixed copies of A and Abar made here, and
+ * the MPO step writes the second GEMM's mixed operand directly. */
+mk_status mk_prepare_transfer_left(mk_ctx ctx, const double* O, const double* A, const double* Abar,
+                                   int64_t Dm, int64_t wl, int64_t Dn, int64_t d, int64_t wr,
+                                   int64_t Dc, int64_t Da, int precisions, mk_op* out);
+mk_status mk_prepare_transfer_right(mk_ctx ctx, const double* O, const double* A, const double* Abar,
+                                    int64_t Da, int64_t d, int64_t Di, int64_t wl, int64_t wr,
+                                    int64_t Dc, int64_t Do, int precisions, mk_op* out);
+
 /* regularize!(v, lvec, rvec) (src/transfermatrix/transfermatrix.jl:70-80), in
  * place, for v (D1 x w x D2) (w = 1: a bond tensor):
  * v[a,g,c] -= (sum_ij lvec[i,j] v[j,g,i]) rvec[a,c];  lvec (D2 x D1), rvec (D1 x D2). */

@@ -48,6 +48,12 @@ mk_status k_tl_mid(const double* t1, const double* O, int64_t Dm, int wl, int d,
 // transfer-right middle step: t2[a,b,k,o] = sum_{h,l} t1[a,h,l,o] O[b,k,h,l]
 mk_status k_tr_mid(const double* t1, const double* O, int64_t Da, int wl, int d, int wr,
                    int64_t Do, double* t2, hipStream_t st);
+// The MPO steps above written as the K-contiguous mixed operand (bf16 hi/lo if
+// split, else fp32) of the transfer's second GEMM, row pitch ldo (zero padded).
+mk_status k_tl_mid_mixed(const double* t1, const double* O, int64_t Dm, int wl, int d, int wr, int64_t Dc,
+                         int64_t ldo, int split, void* hi, void* lo, hipStream_t st);
+mk_status k_tr_mid_mixed(const double* t1, const double* O, int64_t Da, int wl, int d, int wr, int64_t Do,
+                         int64_t ldo, int split, void* hi, void* lo, hipStream_t st);
 
 // physical two-site term without a GEMM: y[a,b,c,e] = alpha*sum P[b,e,h,j] x[a,h,c,j] + beta*y
 mk_status k_phys2(const double* P, const double* x, int64_t Dl, int d1, int64_t Dr, int d2,

@@ -205,6 +205,88 @@ __global__ void tr_mid_kernel(const double* __restrict__ t1, const double* __res
   }
 }
 
+// The MPO step of the left transfer written straight into the K-contiguous
+// mixed-precision operand of the second GEMM (one pass instead of a float64
+// round trip and a conversion):
+//   t2[(m,k),(b,c)] = sum_{g,h} t1[m + Dm (g + wl (h + d c))] O[g + wl (k + d (h + d b))]
+// stored out[(b + wr c) * ldo + m + Dm k], zero for Dm d <= kk < ldo.
+template <bool SPLIT>
+__global__ void tl_mid_mixed_kernel(const double* __restrict__ t1, const double* __restrict__ O, int64_t Dm,
+                                    int wl, int d, int wr, int64_t Dc, int64_t ldo, void* __restrict__ hi,
+                                    void* __restrict__ lo) {
+  const int64_t n = ldo * wr * Dc, K = Dm * d;
+  GRID_STRIDE(idx, n) {
+    const int64_t kk = idx % ldo, col = idx / ldo;
+    const int b = (int)(col % wr);
+    const int64_t c = col / wr;
+    double s = 0.0;
+    if (kk < K) {
+      const int64_t m = kk % Dm;
+      const int k = (int)(kk / Dm);
+      for (int h = 0; h < d; ++h)
+        for (int g = 0; g < wl; ++g) {
+          const double ov = O[g + wl * (k + d * (h + d * b))];
+          if (ov != 0.0) s += t1[m + Dm * (g + (int64_t)wl * (h + d * c))] * ov;
+        }
+    }
+    if constexpr (SPLIT) {
+      __hip_bfloat16 h, l;
+      split2(s, h, l);
+      static_cast<__hip_bfloat16*>(hi)[idx] = h;
+      static_cast<__hip_bfloat16*>(lo)[idx] = l;
+    } else {
+      static_cast<float*>(hi)[idx] = (float)s;
+    }
+  }
+}
+
+// The MPO step of the right transfer written straight into the K-contiguous
+// mixed-precision left operand of the second GEMM (64 x 64 LDS transpose tiles:
+// reads run along a, writes along (k, o)):
+//   t2[(a,b),(k,o)] = sum_{l,h} t1[a + Da (h + d (l + wr o))] O[b + wl (k + d (h + d l))]
+// stored out[(a + Da b) * ldo + k + d o], zero for d Do <= kk < ldo.
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void tr_mid_mixed_kernel(const double* __restrict__ t1,
+                                                           const double* __restrict__ O, int64_t Da, int wl, int d,
+                                                           int wr, int64_t Do, int64_t ldo, void* __restrict__ hi,
+                                                           void* __restrict__ lo) {
+  __shared__ double tile[64][65];
+  const int64_t rows = Da * wl, K = (int64_t)d * Do;
+  const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int cc = ty; cc < 64; cc += 4) {
+    const int64_t r = r0 + tx, kk = c0 + cc;
+    double s = 0.0;
+    if (r < rows && kk < K) {
+      const int64_t a = r % Da;
+      const int b = (int)(r / Da);
+      const int k = (int)(kk % d);
+      const int64_t o = kk / d;
+      for (int l = 0; l < wr; ++l)
+        for (int h = 0; h < d; ++h) {
+          const double ov = O[b + wl * (k + d * (h + d * l))];
+          if (ov != 0.0) s += t1[a + Da * (h + d * (l + (int64_t)wr * o))] * ov;
+        }
+    }
+    tile[cc][tx] = s;
+  }
+  __syncthreads();
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int64_t r = r0 + rr, kk = c0 + tx;
+    if (r < rows && kk < ldo) {
+      const double v = tile[tx][rr];
+      if constexpr (SPLIT) {
+        __hip_bfloat16 h, l;
+        split2(v, h, l);
+        static_cast<__hip_bfloat16*>(hi)[r * ldo + kk] = h;
+        static_cast<__hip_bfloat16*>(lo)[r * ldo + kk] = l;
+      } else {
+        static_cast<float*>(hi)[r * ldo + kk] = (float)v;
+      }
+    }
+  }
+}
+
 __global__ void phys2_kernel(const double* __restrict__ P, const double* __restrict__ x, int64_t Dl,
                              int d1, int64_t Dr, int d2, double alpha, double beta,
                              double* __restrict__ y) {
@@ -667,6 +749,22 @@ mk_status k_tl_mid(const double* t1, const double* O, int64_t Dm, int wl, int d,
 mk_status k_tr_mid(const double* t1, const double* O, int64_t Da, int wl, int d, int wr, int64_t Do,
                    double* t2, hipStream_t st) {
   LAUNCH_1D(tr_mid_kernel, Da * wl * d * Do, st, t1, O, Da, wl, d, wr, Do, t2);
+}
+mk_status k_tl_mid_mixed(const double* t1, const double* O, int64_t Dm, int wl, int d, int wr, int64_t Dc,
+                         int64_t ldo, int split, void* hi, void* lo, hipStream_t st) {
+  if (split) LAUNCH_1D(tl_mid_mixed_kernel<true>, ldo * wr * Dc, st, t1, O, Dm, wl, d, wr, Dc, ldo, hi, lo);
+  LAUNCH_1D(tl_mid_mixed_kernel<false>, ldo * wr * Dc, st, t1, O, Dm, wl, d, wr, Dc, ldo, hi, lo);
+}
+mk_status k_tr_mid_mixed(const double* t1, const double* O, int64_t Da, int wl, int d, int wr, int64_t Do,
+                         int64_t ldo, int split, void* hi, void* lo, hipStream_t st) {
+  if (Da * wl * ldo == 0) return MK_OK;
+  dim3 g((unsigned)cdiv(Da * wl, (int64_t)64), (unsigned)cdiv(ldo, (int64_t)64));
+  if (split)
+    hipLaunchKernelGGL(tr_mid_mixed_kernel<true>, g, dim3(256), 0, st, t1, O, Da, wl, d, wr, Do, ldo, hi, lo);
+  else
+    hipLaunchKernelGGL(tr_mid_mixed_kernel<false>, g, dim3(256), 0, st, t1, O, Da, wl, d, wr, Do, ldo, hi, lo);
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
 }
 mk_status k_phys2(const double* P, const double* x, int64_t Dl, int d1, int64_t Dr, int d2,
                   double alpha, double beta, double* y, hipStream_t st) {

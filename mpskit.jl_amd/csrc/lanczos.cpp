@@ -7,18 +7,26 @@
 //
 // Each restart cycle starts from the current vector x with one float64 apply
 // w = H x, which gives the exact Ritz value theta = <x, w> and residual
-// r = w - theta x.  The cycle then extends span{x} by a Krylov space of the
+// r = w - theta x.  The cycle extends span{x, y_2 .. y_p} (x and the next
+// lowest Ritz vectors kept from the previous cycle: a thick restart, as
+// KrylovKit's Lanczos keeps part of its subspace) by a Krylov space of the
 // (possibly mixed-precision) operator started from r, orthonormalised in
 // float64 (two classical Gram-Schmidt passes), and takes the lowest Ritz pair
-// of the projected matrix whose x-row <q, H x> uses the float64 w.  With a
-// float64 inner operator this is restarted Lanczos; with an FP32 / BF16x3
-// inner operator it is iterative refinement: only the correction is computed
-// in low precision, so every cycle contracts the float64 residual by about the
-// low-precision error and the loop stops on the float64 residual.
+// of the projected matrix, whose x-row <q, H x> uses the float64 w.  Keeping
+// Ritz vectors carries the convergence of the cycles before, which a restart
+// from x alone loses: with eigenvalue gaps small against the spectral width
+// (the local problems of near-critical VUMPS at large D) a single-vector
+// restart stalls.  With a float64 inner operator this is thick-restart
+// Lanczos; with an FP32 / BF16x3 inner operator it is iterative refinement:
+// only the correction is computed in low precision, so every cycle contracts
+// the float64 residual by about the low-precision error and the loop stops on
+// the float64 residual.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
 
+#include "gemm.h"
 #include "kernels.h"
 #include "ops.h"
 
@@ -69,6 +77,27 @@ void jacobi_eig(std::vector<double> A, int n, std::vector<double>& ev, std::vect
   for (int i = 0; i < n; ++i) ev[i] = A[(size_t)i * n + i];
 }
 
+// the p lowest eigenpairs of the leading k x k block of G (row-major, stride
+// ld): eigenvalues ascending in lam, eigenvectors as columns of S (k x p,
+// column-major)
+void lowest_p(const std::vector<double>& G, int ld, int k, int p, std::vector<double>& lam,
+              std::vector<double>& S) {
+  std::vector<double> A((size_t)k * k);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) A[(size_t)i * k + j] = 0.5 * (G[(size_t)i * ld + j] + G[(size_t)j * ld + i]);
+  std::vector<double> ev, Q;
+  jacobi_eig(A, k, ev, Q);
+  std::vector<int> idx(k);
+  for (int i = 0; i < k; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+  lam.resize(p);
+  S.resize((size_t)k * p);
+  for (int c = 0; c < p; ++c) {
+    lam[c] = ev[idx[c]];
+    for (int i = 0; i < k; ++i) S[(size_t)c * k + i] = Q[(size_t)i * k + idx[c]];
+  }
+}
+
 // lowest eigenpair of the leading k x k block of G (row-major, stride ld)
 void lowest(const std::vector<double>& G, int ld, int k, double& lam, std::vector<double>& s) {
   std::vector<double> A((size_t)k * k);
@@ -114,6 +143,11 @@ void lanczos_lowest(const std::vector<double>& G, int ld, int k, double& lam, st
   lowest(G, ld, k, lam, s);
 }
 
+void lanczos_lowest_p(const std::vector<double>& G, int ld, int k, int p, std::vector<double>& lam,
+                      std::vector<double>& S) {
+  lowest_p(G, ld, k, p, lam, S);
+}
+
 // inner Krylov stopping factor for the mixed modes: stop the low-precision
 // cycle once its estimated residual is below eta * (float64 residual at the
 // start of the cycle), i.e. aim for a few digits of contraction per cycle.
@@ -144,18 +178,21 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
   MK_CHECK_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int64_t n = op->x_elems;
-  const int m = krylovdim - 1;  // Krylov vectors per cycle besides x
+  const int K = krylovdim;                        // basis columns: x, kept Ritz vectors, Krylov vectors
+  const int pmax = std::max(1, krylovdim / 2);    // Ritz vectors kept at a restart (x included)
 
-  // basis Q = [x, v_0, ..., v_m] (column-major n x (m + 2)), w = H x, u = work
-  DevBuf Qb, wb, ub, cb, pb;
-  MK_TRY(Qb.alloc((size_t)n * (m + 2) * 8));
+  // basis Q = [x, y_2..y_p, v_0, ...] (column-major n x K), then pmax columns for
+  // the new Ritz vectors; w = H x, u = work
+  DevBuf Qb, wb, ub, cb, pb, sb;
+  MK_TRY(Qb.alloc((size_t)n * (K + pmax) * 8));
   MK_TRY(wb.alloc((size_t)n * 8));
   MK_TRY(ub.alloc((size_t)n * 8));
-  // coefficients: generic dots at [0, m + 4); fused passes at [FC, FC + 2 * 32 + 2)
+  // coefficients: generic dots at [0, K + 4); fused passes at [FC, FC + 2 * 32 + 2)
   constexpr int FC = 1024;
   MK_TRY(cb.alloc((size_t)(FC + 2 * 32 + 2) * 8));
+  MK_TRY(sb.alloc((size_t)K * pmax * 8));
   const int64_t nb = std::max<int64_t>(1, cdiv(n, 256 * 16));
-  MK_TRY(pb.alloc((size_t)std::max<int64_t>(nb * (m + 4), (int64_t)cgs_blocks(n) * 34) * 8));
+  MK_TRY(pb.alloc((size_t)std::max<int64_t>(nb * (K + 4), (int64_t)cgs_blocks(n) * 34) * 8));
   double* Q = static_cast<double*>(Qb.ptr);
   double* w = static_cast<double*>(wb.ptr);
   double* u = static_cast<double*>(ub.ptr);
@@ -173,8 +210,10 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
   double theta = 0.0, res = INFINITY;
   double est_last = INFINITY;
   bool conv = false;
-  const int ldg = m + 2;
-  std::vector<double> G((size_t)ldg * ldg), c, c2, s;
+  const int ldg = K;
+  int p = 1;                        // kept vectors [x, y_2..y_p] at the front of Q
+  std::vector<double> lamk(1, 0.0); // Ritz values of the kept vectors
+  std::vector<double> G((size_t)ldg * ldg), c, c2, s, cx, lamv, S;
   for (int outer = 0;; ++outer) {
     // float64 mode: accept the Ritz estimate of the previous cycle, as
     // KrylovKit does, which saves one apply
@@ -186,34 +225,58 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
     // float64 Ritz value and residual of x
     MK_TRY(op_apply(op, col(0), w, 1.0, 0.0, MK_FP64));
     ++ops;
-    MK_TRY(dv.dots(col(0), n, 1, w, c));
-    theta = c[0];
-    MK_TRY(k_axpby(1.0, w, 0.0, col(1), n, st));
-    MK_TRY(k_axpby(-theta, col(0), 1.0, col(1), n, st));
-    // re-orthogonalise the residual against x (float64)
-    MK_TRY(dv.dots(col(0), n, 1, col(1), c));
-    MK_TRY(dv.subtract(col(0), n, 1, col(1)));
-    MK_TRY(dv.norm(col(1), n, res));
+    MK_TRY(dv.dots(Q, n, p, w, cx));          // <x, w> and <y_i, w>
+    theta = cx[0];
+    MK_TRY(k_axpby(1.0, w, 0.0, col(p), n, st));
+    MK_TRY(k_axpby(-theta, col(0), 1.0, col(p), n, st));
+    // re-orthogonalise the residual against x (float64): its norm is the residual of x
+    MK_TRY(dv.dots(col(0), n, 1, col(p), c));
+    MK_TRY(dv.subtract(col(0), n, 1, col(p)));
+    MK_TRY(dv.norm(col(p), n, res));
     MK_REQUIRE(std::isfinite(theta) && std::isfinite(res), MK_NONFINITE, "eigsolve: non-finite Ritz pair");
     if (res <= tol) {
       conv = true;
       break;
     }
     if (outer == maxiter) break;
-    MK_TRY(k_axpby(1.0 / res, col(1), 0.0, col(1), n, st));
+    // the Krylov space starts from the part of r outside the kept Ritz vectors
+    double rperp = res;
+    if (p > 1) {
+      for (int pass = 0; pass < 2; ++pass) {
+        MK_TRY(dv.dots(Q, n, p, col(p), c));
+        MK_TRY(dv.subtract(Q, n, p, col(p)));
+      }
+      MK_TRY(dv.norm(col(p), n, rperp));
+    }
+    if (!(rperp > 1e-14 * std::max(std::fabs(theta), res))) {
+      // r lies in the kept space: restart from x alone
+      p = 1;
+      MK_TRY(k_axpby(1.0, w, 0.0, col(1), n, st));
+      MK_TRY(k_axpby(-theta, col(0), 1.0, col(1), n, st));
+      MK_TRY(dv.dots(col(0), n, 1, col(1), c));
+      MK_TRY(dv.subtract(col(0), n, 1, col(1)));
+      rperp = res;
+    }
+    MK_TRY(k_axpby(1.0 / rperp, col(p), 0.0, col(p), n, st));
 
     std::fill(G.begin(), G.end(), 0.0);
     G[0] = theta;
-    G[1] = G[ldg] = res;  // <v_0, H x> = |r|
-    int k = 1;            // number of Krylov vectors v_0..v_{k-1} in the cycle
+    for (int i = 1; i < p; ++i) {
+      G[(size_t)i * ldg + i] = lamk[i];
+      G[i] = G[(size_t)i * ldg] = cx[i];       // <y_i, H x> in float64
+    }
+    G[p] = G[(size_t)p * ldg] = rperp;         // <v_0, H x> = |r_perp|
+    int k = 1;                                 // Krylov vectors v_0..v_{k-1} in the cycle
     const double thresh = std::max(tol, inner_eta(precision) * res);
-    for (int j = 0; j < m; ++j) {
+    const int mmax = K - p;                    // room for Krylov vectors
+    for (int j = 0; j < mmax; ++j) {
+      const int cj = p + j;                    // column of v_j
       // u = H_p v_j
-      MK_TRY(op_apply(op, col(j + 1), u, 1.0, 0.0, precision));
+      MK_TRY(op_apply(op, col(cj), u, 1.0, 0.0, precision));
       ++ops;
-      // two classical Gram-Schmidt passes against [x, v_0..v_j]
+      // two classical Gram-Schmidt passes against [x, y.., v_0..v_j]
       double beta = 0.0, uw = 0.0;
-      const int kq = j + 2;
+      const int kq = cj + 1;
       if (kq <= 32) {
         // fused: c1 = Q^T u; u -= Q c1 with c2 = Q^T u in the same read of Q;
         // u -= Q c2 with |u|^2 and <w, u>; one readback for the step
@@ -221,13 +284,13 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
         MK_TRY(k_multidot(Q, n, kq, u, n, cd, dv.partial, st));
         MK_TRY(k_cgs_update_dot(Q, n, kq, cd, u, n, cd + 32, dv.partial, st));
         MK_TRY(k_cgs_update_norm(Q, n, kq, cd + 32, u, w, n, cd + 64, dv.partial, st));
-        std::vector<double> h(2 * 32 + 2);
-        MK_CHECK_HIP(hipMemcpyAsync(h.data(), cd, h.size() * 8, hipMemcpyDeviceToHost, st));
+        std::vector<double> hv(2 * 32 + 2);
+        MK_CHECK_HIP(hipMemcpyAsync(hv.data(), cd, hv.size() * 8, hipMemcpyDeviceToHost, st));
         MK_CHECK_HIP(hipStreamSynchronize(st));
-        c.assign(h.begin(), h.begin() + kq);
-        c2.assign(h.begin() + 32, h.begin() + 32 + kq);
-        beta = std::sqrt(std::max(h[64], 0.0));
-        uw = h[65];
+        c.assign(hv.begin(), hv.begin() + kq);
+        c2.assign(hv.begin() + 32, hv.begin() + 32 + kq);
+        beta = std::sqrt(std::max(hv[64], 0.0));
+        uw = hv[65];
       } else {
         MK_TRY(dv.dots(Q, n, kq, u, c));
         MK_TRY(dv.subtract(Q, n, kq, u));
@@ -238,27 +301,35 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
         MK_TRY(dv.dots(w, n, 1, u, t));
         uw = t[0];
       }
-      // projected matrix among the Krylov vectors; the x-row stays float64 (<v_i, w>)
-      for (int i = 1; i < j + 2; ++i) G[(size_t)i * ldg + (j + 1)] = c[i] + c2[i];
+      // projected matrix; the x-row stays float64 (<v_i, w>)
+      for (int i = 1; i < kq; ++i) G[(size_t)i * ldg + cj] = G[(size_t)cj * ldg + i] = c[i] + c2[i];
       MK_REQUIRE(std::isfinite(beta), MK_NONFINITE, "eigsolve: non-finite Krylov vector");
       double lam;
-      lowest(G, ldg, j + 2, lam, s);
-      est_last = beta * std::fabs(s[j + 1]);
+      lowest(G, ldg, kq, lam, s);
+      est_last = beta * std::fabs(s[cj]);
       k = j + 1;
-      if (est_last <= thresh || beta <= 1e-14 * std::fabs(theta) || j + 1 == m) break;
-      MK_TRY(k_axpby(1.0 / beta, u, 0.0, col(j + 2), n, st));
-      G[(size_t)(j + 2) * ldg + (j + 1)] = beta;
+      if (est_last <= thresh || beta <= 1e-14 * std::fabs(theta) || j + 1 == mmax) break;
+      MK_TRY(k_axpby(1.0 / beta, u, 0.0, col(cj + 1), n, st));
+      G[(size_t)(cj + 1) * ldg + cj] = G[(size_t)cj * ldg + cj + 1] = beta;
       // <v_{j+1}, w> = <u, w> / beta
-      G[(size_t)(j + 2)] = G[(size_t)(j + 2) * ldg] = uw / beta;
+      G[(size_t)(cj + 1)] = G[(size_t)(cj + 1) * ldg] = uw / beta;
     }
-    // lowest Ritz vector of span{x, v_0..v_{k-1}} becomes the new x
-    double lam;
-    lowest(G, ldg, k + 1, lam, s);
-    MK_CHECK_HIP(hipMemcpyAsync(dv.coef, s.data(), (size_t)(k + 1) * 8, hipMemcpyHostToDevice, st));
-    MK_TRY(k_gemv(Q, n, k + 1, dv.coef, u, n, st));
-    MK_TRY(dv.norm(u, n, nrm));
-    MK_TRY(k_axpby(1.0 / nrm, u, 0.0, col(0), n, st));
-    theta = lam;
+    // the lowest Ritz vectors of the cycle's space: x and the kept vectors of
+    // the next cycle
+    const int kk = p + k;
+    const int pn = std::min(pmax, kk);
+    lowest_p(G, ldg, kk, pn, lamv, S);
+    MK_CHECK_HIP(hipMemcpyAsync(sb.ptr, S.data(), S.size() * 8, hipMemcpyHostToDevice, st));
+    ProbSpec ps;
+    ps.M = n; ps.N = pn; ps.C = col(K); ps.ldc = n;
+    ps.add(Q, n, sb.ptr, kk, kk);
+    MK_TRY(gemm_f64(&ps, 1, GemmShape{}, st));
+    MK_CHECK_HIP(hipMemcpyAsync(col(0), col(K), (size_t)n * pn * 8, hipMemcpyDeviceToDevice, st));
+    MK_TRY(dv.norm(col(0), n, nrm));
+    MK_TRY(k_axpby(1.0 / nrm, col(0), 0.0, col(0), n, st));
+    p = pn;
+    lamk = lamv;
+    theta = lamv[0];
   }
   MK_CHECK_HIP(hipMemcpyAsync(x, col(0), (size_t)n * 8, hipMemcpyDeviceToDevice, st));
   MK_CHECK_HIP(hipStreamSynchronize(st));

@@ -97,7 +97,7 @@ mk_status make_all_mixed(Op* op) {
   hipStream_t st = op->ctx->stream;
   const int pm = op->precisions & (MK_PREC_BF16X3 | MK_PREC_FP32);
   if (!pm) return MK_OK;
-  OpMat* mats[] = {&op->gen.L, &op->gen.R, &op->IC, &op->BE, &op->CA, &op->Cp, &op->Bp};
+  OpMat* mats[] = {&op->gen.L, &op->gen.R, &op->IC, &op->BE, &op->CA, &op->Cp, &op->Bp, &op->TA, &op->TB};
   for (OpMat* m : mats)
     if (m->present()) MK_TRY(m->make_mixed(pm, st));
   for (OpMat& m : op->AB) MK_TRY(m.make_mixed(pm, st));
@@ -408,6 +408,109 @@ mk_status apply_ac_jordan(Op* op, const double* x, double* y, double alpha, doub
   return MK_OK;
 }
 
+// Left transfer y = alpha x T + beta y (transfer_left, transfer.jl:125-127 for
+// an MPO, :18-20 for an MPS): t1 = x A over the ket's left bond, the MPO step,
+// y = Abar^T t2.  In FP32 / BF16x3 the two GEMMs run on the prepared mixed
+// copies of A and Abar with x converted on the way in, and the MPO step writes
+// the second GEMM's mixed operand directly; the MPS transfer has no MPO step
+// (t1 is already t2).
+mk_status apply_transfer_left(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  Ctx* ctx = op->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t Dm = op->tD[0], Dn = op->tD[1], Dc = op->tD[2], Da = op->tD[3];
+  const int wl = op->tw[0], d = op->tw[1], wr = op->tw[2];
+  const double* O = static_cast<const double*>(op->TO.ptr);
+  const int64_t M1 = Dm * wl, N1 = (int64_t)d * Dc, K2 = Dm * d, N2 = (int64_t)wr * Dc;
+  if (prec == MK_FP64) {
+    double *t1 = nullptr, *t2 = nullptr;
+    MK_TRY(carve(ctx, {{(size_t)M1 * N1 * 8, (void**)&t1}, {op->tmpo ? (size_t)K2 * N2 * 8 : 0, (void**)&t2}}));
+    ProbSpec p1;
+    p1.M = M1; p1.N = N1; p1.C = t1; p1.ldc = M1;
+    p1.add(x, M1, op->TA.f64(), Dn, Dn);
+    MK_TRY(gemm_f64(&p1, 1, GemmShape{}, st));
+    if (op->tmpo) MK_TRY(k_tl_mid(t1, O, Dm, wl, d, wr, Dc, t2, st));
+    else t2 = t1;
+    ProbSpec p2;
+    p2.M = Da; p2.N = N2; p2.C = y; p2.ldc = Da; p2.alpha = alpha; p2.beta = beta;
+    p2.add(op->TB.f64(), K2, t2, K2, K2);
+    return gemm_f64(&p2, 1, GemmShape{true, false}, st);
+  }
+  const bool bf = prec == MK_BF16X3;
+  double* t1 = nullptr;
+  void *mx = nullptr, *mt = nullptr;
+  MK_TRY(carve(ctx, {{(size_t)M1 * N1 * 8, (void**)&t1},
+                     {view_bytes(M1, Dn, 'A', prec), &mx},
+                     {view_bytes(K2, N2, 'B', prec), &mt}}));
+  MixedView vx, vt;
+  MK_TRY(convert_view(ctx, x, M1, Dn, M1, 'A', prec, &vx, mx));
+  ProbSpec p1;
+  p1.M = M1; p1.N = N1; p1.C = t1; p1.ldc = M1;
+  if (bf) p1.add(vx.hi, vx.kpad, op->TA.hi.ptr, op->TA.kpad, Dn, vx.lo, op->TA.lo.ptr);
+  else p1.add(vx.hi, vx.kpad, op->TA.f32.ptr, op->TA.kpad, Dn);
+  MK_TRY(run_gemm(&p1, 1, prec, GemmShape{}, st));
+  if (op->tmpo) {
+    vt = view_layout(K2, N2, 'B', prec, mt);
+    MK_TRY(k_tl_mid_mixed(t1, O, Dm, wl, d, wr, Dc, vt.kpad, bf ? 1 : 0, vt.hi, vt.lo, st));
+  } else {
+    MK_TRY(convert_view(ctx, t1, K2, N2, K2, 'B', prec, &vt, mt));
+  }
+  ProbSpec p2;
+  p2.M = Da; p2.N = N2; p2.C = y; p2.ldc = Da; p2.alpha = alpha; p2.beta = beta;
+  if (bf) p2.add(op->TB.hi.ptr, op->TB.kpad, vt.hi, vt.kpad, K2, op->TB.lo.ptr, vt.lo);
+  else p2.add(op->TB.f32.ptr, op->TB.kpad, vt.hi, vt.kpad, K2);
+  return run_gemm(&p2, 1, prec, GemmShape{}, st);
+}
+
+// Right transfer y = alpha T v + beta y (transfer_right, transfer.jl:128-130
+// for an MPO, :21-23 for an MPS): t1 = A v over the ket's right bond, the MPO
+// step, y = t2 Abar^T; mixed precisions as for the left transfer.
+mk_status apply_transfer_right(Op* op, const double* v, double* y, double alpha, double beta, int prec) {
+  Ctx* ctx = op->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t Da = op->tD[0], Di = op->tD[1], Dc = op->tD[2], Do = op->tD[3];
+  const int wl = op->tw[0], d = op->tw[1], wr = op->tw[2];
+  const double* O = static_cast<const double*>(op->TO.ptr);
+  const int64_t M1 = Da * d, N1 = (int64_t)wr * Do, M2 = Da * wl, K2 = (int64_t)d * Do;
+  if (prec == MK_FP64) {
+    double *t1 = nullptr, *t2 = nullptr;
+    MK_TRY(carve(ctx, {{(size_t)M1 * N1 * 8, (void**)&t1}, {op->tmpo ? (size_t)M2 * K2 * 8 : 0, (void**)&t2}}));
+    ProbSpec p1;
+    p1.M = M1; p1.N = N1; p1.C = t1; p1.ldc = M1;
+    p1.add(op->TA.f64(), M1, v, Di, Di);
+    MK_TRY(gemm_f64(&p1, 1, GemmShape{}, st));
+    if (op->tmpo) MK_TRY(k_tr_mid(t1, O, Da, wl, d, wr, Do, t2, st));
+    else t2 = t1;
+    ProbSpec p2;
+    p2.M = M2; p2.N = Dc; p2.C = y; p2.ldc = M2; p2.alpha = alpha; p2.beta = beta;
+    p2.add(t2, M2, op->TB.f64(), Dc, K2);
+    return gemm_f64(&p2, 1, GemmShape{false, true}, st);
+  }
+  const bool bf = prec == MK_BF16X3;
+  double* t1 = nullptr;
+  void *mv = nullptr, *mt = nullptr;
+  MK_TRY(carve(ctx, {{(size_t)M1 * N1 * 8, (void**)&t1},
+                     {view_bytes(Di, N1, 'B', prec), &mv},
+                     {view_bytes(M2, K2, 'A', prec), &mt}}));
+  MixedView vv, vt;
+  MK_TRY(convert_view(ctx, v, Di, N1, Di, 'B', prec, &vv, mv));
+  ProbSpec p1;
+  p1.M = M1; p1.N = N1; p1.C = t1; p1.ldc = M1;
+  if (bf) p1.add(op->TA.hi.ptr, op->TA.kpad, vv.hi, vv.kpad, Di, op->TA.lo.ptr, vv.lo);
+  else p1.add(op->TA.f32.ptr, op->TA.kpad, vv.hi, vv.kpad, Di);
+  MK_TRY(run_gemm(&p1, 1, prec, GemmShape{}, st));
+  if (op->tmpo) {
+    vt = view_layout(M2, K2, 'A', prec, mt);
+    MK_TRY(k_tr_mid_mixed(t1, O, Da, wl, d, wr, Do, vt.kpad, bf ? 1 : 0, vt.hi, vt.lo, st));
+  } else {
+    MK_TRY(convert_view(ctx, t1, M2, K2, M2, 'A', prec, &vt, mt));
+  }
+  ProbSpec p2;
+  p2.M = M2; p2.N = Dc; p2.C = y; p2.ldc = M2; p2.alpha = alpha; p2.beta = beta;
+  if (bf) p2.add(vt.hi, vt.kpad, op->TB.hi.ptr, op->TB.kpad, K2, vt.lo, op->TB.lo.ptr);
+  else p2.add(vt.hi, vt.kpad, op->TB.f32.ptr, op->TB.kpad, K2);
+  return run_gemm(&p2, 1, prec, GemmShape{}, st);
+}
+
 }  // namespace
 
 mk_status op_apply(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
@@ -425,6 +528,10 @@ mk_status op_apply(Op* op, const double* x, double* y, double alpha, double beta
       return apply_ac2_jordan(op, x, y, alpha, beta, prec);
     case OP_AC_JOR:
       return apply_ac_jordan(op, x, y, alpha, beta, prec);
+    case OP_TL:
+      return apply_transfer_left(op, x, y, alpha, beta, prec);
+    case OP_TR:
+      return apply_transfer_right(op, x, y, alpha, beta, prec);
   }
   return MK_NOT_SUPPORTED;
 }
@@ -974,6 +1081,80 @@ extern "C" mk_status mk_prepare_ac_jordan(mk_ctx h, const double* GL, const doub
   op->x_elems = Dl * d * Dr;
   op->y_elems = Dl * d * Dr;
   op->flops = fl;
+  return finish(op, out);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: prepared transfer matrices
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Copies of A and Abar (and O) owned by the operator, so that later changes
+// of the caller's tensors do not reach it.
+mk_status own_copy(OpMat& m, char role, const double* src, int64_t rows, int64_t cols, hipStream_t st) {
+  MK_TRY(m.alloc(role, rows, cols));
+  MK_CHECK_HIP(hipMemcpyAsync(m.f64(), src, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, st));
+  return MK_OK;
+}
+
+}  // namespace
+
+extern "C" mk_status mk_prepare_transfer_left(mk_ctx h, const double* O, const double* A, const double* Abar,
+                                              int64_t Dm, int64_t wl, int64_t Dn, int64_t d, int64_t wr, int64_t Dc,
+                                              int64_t Da, int precisions, mk_op* out) {
+  MK_REQUIRE(okdim(Dm) && okdim(wl) && okdim(Dn) && okdim(d) && okdim(wr) && okdim(Dc) && okdim(Da), MK_BAD_SHAPE,
+             "prepare_transfer_left: non-positive dimension");
+  MK_REQUIRE(A && Abar, MK_BAD_ARG, "prepare_transfer_left: null pointer");
+  MK_REQUIRE(O || (wl == 1 && wr == 1), MK_BAD_SHAPE, "prepare_transfer_left: an MPS transfer has wl = wr = 1");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_TL, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  // A (Dn x d Dc) is the right factor of t1 = x A; Abar (Dm d x Da), K = Dm d
+  // fast, is stored as a right factor too, which makes its mixed copy the
+  // K-contiguous left factor Abar^T of y = Abar^T t2
+  OP_TRY(own_copy(op->TA, 'B', A, Dn, d * Dc, st));
+  OP_TRY(own_copy(op->TB, 'B', Abar, Dm * d, Da, st));
+  if (O) {
+    op->tmpo = true;
+    OP_TRY(op->TO.alloc((size_t)wl * d * d * wr * 8));
+    hipError_t e = hipMemcpyAsync(op->TO.ptr, O, (size_t)wl * d * d * wr * 8, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) { drop(op); set_error("prepare_transfer_left: copy failed"); return MK_HIP_ERROR; }
+  }
+  op->tD[0] = Dm; op->tD[1] = Dn; op->tD[2] = Dc; op->tD[3] = Da;
+  op->tw[0] = (int)wl; op->tw[1] = (int)d; op->tw[2] = (int)wr;
+  op->x_elems = Dm * wl * Dn;
+  op->y_elems = Da * wr * Dc;
+  op->flops = 2.0 * Dm * wl * Dn * d * Dc + 2.0 * Da * Dm * d * wr * Dc;
+  return finish(op, out);
+}
+
+extern "C" mk_status mk_prepare_transfer_right(mk_ctx h, const double* O, const double* A, const double* Abar,
+                                               int64_t Da, int64_t d, int64_t Di, int64_t wl, int64_t wr, int64_t Dc,
+                                               int64_t Do, int precisions, mk_op* out) {
+  MK_REQUIRE(okdim(Da) && okdim(d) && okdim(Di) && okdim(wl) && okdim(wr) && okdim(Dc) && okdim(Do), MK_BAD_SHAPE,
+             "prepare_transfer_right: non-positive dimension");
+  MK_REQUIRE(A && Abar, MK_BAD_ARG, "prepare_transfer_right: null pointer");
+  MK_REQUIRE(O || (wl == 1 && wr == 1), MK_BAD_SHAPE, "prepare_transfer_right: an MPS transfer has wl = wr = 1");
+  Op* op = nullptr;
+  MK_TRY(new_op(h, OP_TR, precisions, &op));
+  hipStream_t st = op->ctx->stream;
+  // A (Da d x Di) is the left factor of t1 = A v; Abar (Dc x d Do) is stored
+  // as a left factor, which makes its mixed copy the K-contiguous right factor
+  // Abar^T of y = t2 Abar^T
+  OP_TRY(own_copy(op->TA, 'A', A, Da * d, Di, st));
+  OP_TRY(own_copy(op->TB, 'A', Abar, Dc, d * Do, st));
+  if (O) {
+    op->tmpo = true;
+    OP_TRY(op->TO.alloc((size_t)wl * d * d * wr * 8));
+    hipError_t e = hipMemcpyAsync(op->TO.ptr, O, (size_t)wl * d * d * wr * 8, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) { drop(op); set_error("prepare_transfer_right: copy failed"); return MK_HIP_ERROR; }
+  }
+  op->tD[0] = Da; op->tD[1] = Di; op->tD[2] = Dc; op->tD[3] = Do;
+  op->tw[0] = (int)wl; op->tw[1] = (int)d; op->tw[2] = (int)wr;
+  op->x_elems = Di * wr * Do;
+  op->y_elems = Da * wl * Dc;
+  op->flops = 2.0 * Da * d * Di * wr * Do + 2.0 * Da * wl * d * Do * Dc;
   return finish(op, out);
 }
 

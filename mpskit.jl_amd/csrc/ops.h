@@ -77,7 +77,7 @@ MixedView view_layout(int64_t rows, int64_t cols, char role, int prec, void* mem
 size_t view_bytes(int64_t rows, int64_t cols, char role, int prec);
 mk_status run_gemm(const ProbSpec* p, int np, int prec, GemmShape sh, hipStream_t st);
 
-enum OpKind { OP_C = 0, OP_AC_GEN = 1, OP_AC2_GEN = 2, OP_AC_JOR = 3, OP_AC2_JOR = 4 };
+enum OpKind { OP_C = 0, OP_AC_GEN = 1, OP_AC2_GEN = 2, OP_AC_JOR = 3, OP_AC2_JOR = 4, OP_TL = 5, OP_TR = 6 };
 
 // Two-step "L (x R)" block: xR = x (Mx x Ki) * R (Ki x nk*Ny);  y = L (Mo x Mx*nk) * xR.
 struct LxR {
@@ -107,6 +107,16 @@ struct Op {
 
   // Jordan one-site terms
   OpMat Cp, Bp;
+
+  // transfer matrices (OP_TL, OP_TR): the ket tensor TA and the bra tensor TB
+  // as GEMM operands, the MPO tensor (none for an MPS transfer) and the sizes
+  //   left:  x (Dm, wl, Dn) -> y (Da, wr, Dc), A (Dn, d, Dc), Abar (Dm, d, Da)
+  //   right: v (Di, wr, Do) -> y (Da, wl, Dc), A (Da, d, Di), Abar (Dc, d, Do)
+  OpMat TA, TB;
+  DevBuf TO;
+  bool tmpo = false;
+  int64_t tD[4] = {0, 0, 0, 0};     // left: Dm, Dn, Dc, Da;  right: Da, Di, Dc, Do
+  int tw[3] = {1, 1, 1};            // wl, d, wr
 
   // U(1) charge blocks of a nearest-neighbour two-site operator: when set,
   // applies only compute the output blocks that conserve charge, each a sum of

@@ -13,8 +13,8 @@ from .derivatives import (
     MPO_AC_Hamiltonian, MPO_C_Hamiltonian, MPODerivativeOperator, PreparedDerivative, prepare_operator,
 )
 from .transfer import (
-    ProductTransferMatrix, RegTransferMatrix, SingleTransferMatrix, TransferMatrix, regularize, regularize_,
-    transfer_left, transfer_right,
+    ProductTransferMatrix, RegTransferMatrix, SingleTransferMatrix, TransferMatrix, prepare_transfer_left,
+    prepare_transfer_right, regularize, regularize_, transfer_left, transfer_right,
 )
 
 __all__ = [
@@ -23,7 +23,7 @@ __all__ = [
     "MPO_AC2_Hamiltonian", "MPO_AC_Hamiltonian", "MPO_C_Hamiltonian", "MPODerivativeOperator",
     "PreparedDerivative", "prepare_operator", "ProductTransferMatrix", "RegTransferMatrix",
     "JordanMPO_AC2_Terms", "JordanMPO_AC_Terms",
-    "SingleTransferMatrix", "TransferMatrix", "regularize", "regularize_", "transfer_left", "transfer_right",
+    "SingleTransferMatrix", "TransferMatrix", "prepare_transfer_left", "prepare_transfer_right", "regularize", "regularize_", "transfer_left", "transfer_right",
 ]
 from .algorithms import (
     DMRG, DMRG2, AdaptiveKrylov, Davidson, Lanczos, expectation_value, find_groundstate, find_groundstate_batched,

@@ -100,6 +100,8 @@ _SIGS = {
     "mk_op_free": ([_P], _INT),
     "mk_transfer_left_mps": ([_P, _P, _P, _P] + [_I64] * 5 + [_P], _INT),
     "mk_transfer_right_mps": ([_P, _P, _P, _P] + [_I64] * 5 + [_P], _INT),
+    "mk_prepare_transfer_left": ([_P, _P, _P, _P] + [_I64] * 7 + [_INT, C.POINTER(_P)], _INT),
+    "mk_prepare_transfer_right": ([_P, _P, _P, _P] + [_I64] * 7 + [_INT, C.POINTER(_P)], _INT),
     "mk_transfer_left_mpo": ([_P, _P, _P, _P, _P] + [_I64] * 7 + [_P], _INT),
     "mk_transfer_right_mpo": ([_P, _P, _P, _P, _P] + [_I64] * 7 + [_P], _INT),
     "mk_regularize": ([_P, _P, _P, _P, _I64, _I64, _I64], _INT),

@@ -8,8 +8,16 @@ its lowest eigenvectors are the exact eigenstates.  The fusing tensors start
 as identities (fused index l + D_l p on the left, p + d r on the right) and
 are brought to mixed-canonical form, which keeps them full-rank isometries.
 The lowest state comes from the library's eigensolver
-(``mk_eigsolve``); further states come from the same derivative with the states
-already found shifted up.
+(``mk_eigsolve``); further states come from the same derivative with the
+states already found moved to the top of the spectrum, P H P + sigma U U^T with
+P = 1 - U U^T and sigma above the largest eigenvalue (a Lanczos estimate of it,
+plus a margin).  Shifting them to the bottom would fail when the ground energy
+lies near zero or the gap is large, and projecting alone is not enough:
+round-off leaves components along U in the Krylov vectors, and Lanczos's
+recurrence amplifies a component whose eigenvalue lies below the rest (the
+found states of P H P sit at 0) until the solver returns a state already
+found.  At the top of the spectrum the same components are harmless to a
+search for the lowest eigenvalue.
 """
 from __future__ import annotations
 
@@ -49,6 +57,23 @@ def _fusing_tensors(H, d, seed):
     return [T.from_numpy(t) for t in tensors], mid
 
 
+def _project(v, found):
+    """v - U U^T v for the orthonormal found states U (two Gram-Schmidt passes)."""
+    for _ in range(2):
+        for u in found:
+            v = v - torch.dot(u.reshape(-1), v.reshape(-1)) * u
+    return v
+
+
+def _above_spectrum(op, x0, e0):
+    """A shift above the spectrum of op: the largest Ritz value of a short
+    Lanczos run on -op (a lower bound of the largest eigenvalue, close to it
+    since it is extreme), plus a margin of the spread and 1."""
+    lam, _, _ = lanczos_min(lambda v: -op(T.as_colmajor(v)), T.as_colmajor(x0), tol=1e-6, krylovdim=30, maxiter=3)
+    top = -lam
+    return top + 0.1 * abs(top - e0) + 1.0
+
+
 def exact_diagonalization(H, num=1, tol=1e-12, krylovdim=30, maxiter=200, seed=0):
     """The ``num`` lowest eigenpairs of a finite chain as (values, FiniteMPS
     states) (exact_diagonalization, ED.jl:1-90, which = :SR).  All states share
@@ -59,23 +84,24 @@ def exact_diagonalization(H, num=1, tol=1e-12, krylovdim=30, maxiter=200, seed=0
     envs = FiniteEnvironments(base, H)
     op = prepare_operator(JordanMPO_AC_Hamiltonian(envs.leftenv(mid, base), H[mid], envs.rightenv(mid, base)))
     vals, states, found = [], [], []
+    sigma = None
     try:
         for _ in range(num):
             if not found:
                 lam, x, _ = op.eigsolve(base.AC, krylovdim=krylovdim, maxiter=maxiter, tol=tol)
             else:
-                shift = 2.0 * (abs(vals[0]) + 1.0)
+                if sigma is None:
+                    sigma = _above_spectrum(op, base.AC, vals[0])
 
                 def apply(v):
-                    y = op(T.as_colmajor(v))
+                    y = _project(op(T.as_colmajor(_project(v, found))), found)
                     for u in found:
-                        y = y + (shift * torch.dot(u.reshape(-1), v.reshape(-1))) * u
+                        y = y + (sigma * torch.dot(u.reshape(-1), v.reshape(-1))) * u
                     return y
 
-                start = base.AC
-                for u in found:
-                    start = start - torch.dot(u.reshape(-1), start.reshape(-1)) * u
+                start = _project(base.AC, found)
                 lam, x, _ = lanczos_min(apply, T.as_colmajor(start), tol=tol, krylovdim=krylovdim, maxiter=maxiter)
+                x = _project(x, found)
             x = T.as_colmajor(x / torch.linalg.vector_norm(x))
             found.append(x)
             vals.append(float(lam))

@@ -18,6 +18,9 @@ contractions at the ends and the SVD of C are torch float64 calls.
 """
 from __future__ import annotations
 
+import math
+from fractions import Fraction
+
 import numpy as np
 import torch
 
@@ -189,23 +192,61 @@ def transfer_spectrum(psi, howmany=20, krylovdim=None, seed=0):
     return ev[order], res
 
 
+def _simplest_rational(lo, hi):
+    """The fraction of smallest denominator in the closed interval [lo, hi]
+    (0 <= lo <= hi, exact Fractions), by the continued-fraction recursion."""
+    fl = math.floor(lo)
+    if fl == lo:
+        return Fraction(fl)
+    if fl + 1 <= hi:
+        return Fraction(fl + 1)
+    inner = _simplest_rational(1 / (hi - fl), 1 / (lo - fl))
+    return fl + 1 / inner
+
+
+def rationalize(x, tol):
+    """The simplest fraction within ``tol`` of ``x`` (Julia's ``rationalize(x;
+    tol)`` as used by approx_angles, transfer_spectrum.jl:107-116)."""
+    x, tol = Fraction(float(x)), Fraction(float(tol))
+    lo, hi = x - tol, x + tol
+    if lo <= 0 <= hi:
+        return Fraction(0)
+    if hi < 0:
+        return -_simplest_rational(-hi, -lo)
+    return _simplest_rational(lo, hi)
+
+
+def approx_angles(spectrum, tol_angle=0.1):
+    """Angles of the eigenvalues snapped to the simplest fraction of pi within
+    ``tol_angle`` (in units of pi), -pi mapped to pi (approx_angles,
+    transfer_spectrum.jl:107-116).  Returned as exact fractions of pi."""
+    out = []
+    for lam in np.asarray(spectrum, dtype=np.complex128):
+        a = rationalize(np.angle(lam) / np.pi, tol_angle)
+        out.append(Fraction(1) if a == -1 else a)
+    return out
+
+
 def marek_gap(spectrum, tol_angle=0.1):
     """(epsilon, delta, theta) of a transfer spectrum (marek_gap, transfer_spectrum.jl:136-160):
-    drop the eigenvalues of modulus 1, take the angle of the largest remaining one,
-    epsilon = -log|lambda| of the first, delta the gap to the next at that angle."""
-    spec = np.asarray(spectrum)
+    drop the eigenvalues of modulus 1, snap the angles to fractions of pi
+    (``approx_angles``), take the angle theta of the largest remaining
+    eigenvalue and the eigenvalues at exactly that snapped angle; epsilon =
+    -log|lambda| of the first of them, delta the difference of the first two
+    -log|lambda| when more than two share the angle (else inf), as the
+    reference has it."""
+    spec = np.asarray(spectrum, dtype=np.complex128)
     spec = spec[np.abs(spec) < 1 - 1e-12]
     if spec.size == 0:
         raise ValueError("marek_gap: no eigenvalue below modulus 1 (request more eigenvalues, or the MPS "
                          "is not injective)")
-    spec = spec[np.argsort(-np.abs(spec))]
-    angles = np.angle(spec)
+    spec = spec[np.argsort(-np.abs(spec), kind="stable")]
+    angles = approx_angles(spec, tol_angle)
     theta = angles[0]
-    # same angle: within tol_angle in units of pi, as approx_angles rationalises angle / pi
-    at = spec[np.abs(np.angle(np.exp(1j * (angles - theta)))) / np.pi < tol_angle]
+    at = spec[[a == theta for a in angles]]
     lambdas = -np.log(np.abs(at))
-    delta = lambdas[1] - lambdas[0] if len(lambdas) > 1 else np.inf
-    return float(lambdas[0]), float(delta), float(theta)
+    delta = lambdas[1] - lambdas[0] if len(lambdas) > 2 else np.inf
+    return float(lambdas[0]), float(delta), float(theta) * np.pi
 
 
 def correlation_length(psi, **kw):

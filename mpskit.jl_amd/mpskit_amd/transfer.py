@@ -41,6 +41,56 @@ def transfer_right(v, *args):
     return _tr_mps(v, A, Ab)
 
 
+def prepare_transfer_left(A, Ab, O=None, precisions=None):
+    """The map x -> transfer_left(x, [O,] A, Ab) prepared for repeated applies
+    (``mk_prepare_transfer_left``), e.g. the fixed-point solves of the infinite
+    environments (infinite_envs.jl:207-346): a ``PreparedDerivative`` whose
+    call takes ``precision`` ("fp64", "fp32", "bf16x3").  x is (Dm, Dn) for an
+    MPS transfer (O None) and (Dm, wl, Dn) for an MPO transfer."""
+    import ctypes as C
+    from .derivatives import PreparedDerivative, _mask
+    A, Ab = T.as_colmajor(A), T.as_colmajor(Ab)
+    Dn, d, Dc = A.shape
+    Dm, d2, Da = Ab.shape
+    _shape_check(d == d2, "prepare_transfer_left: physical spaces differ")
+    if O is None:
+        wl = wr = 1
+        xs, ys = (Dm, Dn), (Da, Dc)
+    else:
+        O = T.as_colmajor(O)
+        wl, dO, dO2, wr = O.shape
+        _shape_check(dO == d == dO2, "prepare_transfer_left: MPO physical spaces differ")
+        xs, ys = (Dm, wl, Dn), (Da, wr, Dc)
+    ctx, h, mask = default_context(), C.c_void_p(), _mask(precisions)
+    check(load().mk_prepare_transfer_left(ctx.handle, None if O is None else T.ptr(O), T.ptr(A), T.ptr(Ab),
+                                          Dm, wl, Dn, d, wr, Dc, Da, mask, C.byref(h)))
+    return PreparedDerivative(h, ctx, xs, ys, "TransferLeft", mask)
+
+
+def prepare_transfer_right(A, Ab, O=None, precisions=None):
+    """The map v -> transfer_right(v, [O,] A, Ab) prepared for repeated applies
+    (``mk_prepare_transfer_right``); v is (Di, Do) for an MPS transfer and
+    (Di, wr, Do) for an MPO transfer."""
+    import ctypes as C
+    from .derivatives import PreparedDerivative, _mask
+    A, Ab = T.as_colmajor(A), T.as_colmajor(Ab)
+    Da, d, Di = A.shape
+    Dc, d2, Do = Ab.shape
+    _shape_check(d == d2, "prepare_transfer_right: physical spaces differ")
+    if O is None:
+        wl = wr = 1
+        xs, ys = (Di, Do), (Da, Dc)
+    else:
+        O = T.as_colmajor(O)
+        wl, dO, dO2, wr = O.shape
+        _shape_check(dO == d == dO2, "prepare_transfer_right: MPO physical spaces differ")
+        xs, ys = (Di, wr, Do), (Da, wl, Dc)
+    ctx, h, mask = default_context(), C.c_void_p(), _mask(precisions)
+    check(load().mk_prepare_transfer_right(ctx.handle, None if O is None else T.ptr(O), T.ptr(A), T.ptr(Ab),
+                                           Da, d, Di, wl, wr, Dc, Do, mask, C.byref(h)))
+    return PreparedDerivative(h, ctx, xs, ys, "TransferRight", mask)
+
+
 def _tl_mps(v, A, Ab):
     Dm, Dn = v.shape
     Dn2, d, Dc = A.shape

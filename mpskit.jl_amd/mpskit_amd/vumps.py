@@ -27,7 +27,7 @@ import torch
 from . import tensors as T
 from .derivatives import JordanMPO_AC_Hamiltonian, MPO_AC_Hamiltonian, MPO_C_Hamiltonian, prepare_operator
 from .states import lq_right, pos_qr, qr_left
-from .transfer import transfer_left, transfer_right
+from .transfer import prepare_transfer_left, prepare_transfer_right, transfer_left, transfer_right
 
 
 def _dot(a, b):
@@ -52,11 +52,19 @@ def _unflat(f, shape):
     return f.reshape(tuple(reversed(shape))).permute(*reversed(range(len(shape))))
 
 
-def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50):
+def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50, apply_inner=None, eta=0.0):
     """Solve apply(x) = b (KrylovKit.linsolve with GMRES, used at infinite_envs.jl:236-324).
     Returns (x, info) with info = dict(normres, numops, converged).  The Krylov
     basis is one (krylovdim + 1) x n GPU matrix, orthogonalised by two classical
-    Gram-Schmidt passes per step with one readback of the Hessenberg column."""
+    Gram-Schmidt passes per step with one readback of the Hessenberg column.
+
+    With ``apply_inner`` (a cheaper, less accurate version of the same map, e.g.
+    its FP32 / BF16x3 apply) the Krylov steps use it and every restart
+    recomputes the residual with ``apply``: each cycle is one step of iterative
+    refinement, the cycle ends once its own residual is below ``eta`` times
+    the cycle's starting residual, and convergence is judged on the residual
+    of ``apply`` alone."""
+    inner = apply if apply_inner is None else apply_inner
     shape = tuple(b.shape)
     bf = _flat(b)
     x = _flat(x0).clone()
@@ -71,8 +79,9 @@ def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50):
         Vb[0] = r / beta
         H = np.zeros((krylovdim + 1, krylovdim))
         y = None
+        goal = max(tol, eta * beta)
         for j in range(krylovdim):
-            w = _flat(apply(_unflat(Vb[j], shape)))
+            w = _flat(inner(_unflat(Vb[j], shape)))
             ops += 1
             Vj = Vb[: j + 1]
             h1 = Vj @ w
@@ -86,7 +95,7 @@ def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50):
             e1[0] = beta
             y, *_ = np.linalg.lstsq(H[: j + 2, : j + 1], e1, rcond=None)
             res = float(np.linalg.norm(H[: j + 2, : j + 1] @ y - e1))
-            if res <= tol or H[j + 1, j] <= 1e-14 * bnorm:
+            if res <= goal or H[j + 1, j] <= 1e-14 * bnorm:
                 break
             Vb[j + 1] = w / H[j + 1, j]
         yt = torch.from_numpy(np.ascontiguousarray(y)).to(Vb.device)
@@ -174,18 +183,21 @@ def _pos_qr(M):
 
 
 def uniform_leftorth(A, C0, tol=1e-14, maxiter=200, eig_miniter=10):
-    """AL[i] C[i] = C[i-1] A[i] (uniform_leftorth!, ortho.jl:215-272)."""
+    """AL[i] C[i] = C[i-1] A[i] (uniform_leftorth!, ortho.jl:215-272).  The error
+    of an iteration is the change of C over the QR sweep, after the fixed-point
+    step (ortho.jl:241-244)."""
     N = len(A)
     C = [None] * N
     C[-1] = C0 / _norm(C0)
     AL = [None] * N
     eps = math.inf
     for it in range(maxiter):
-        C_before = C[-1]
         if it >= eig_miniter:
-            _, vec = arnoldi_dominant(lambda v: _left_cell(v, A, AL), C[-1], tol=max(eps * eps, 1e-15))
+            _, vec = arnoldi_dominant(lambda v: _left_cell(v, A, AL), C[-1], tol=max(eps * eps, 1e-15),
+                                      krylovdim=30, maxiter=200)
             _, R = _pos_qr(vec)
             C[-1] = R
+        C_before = C[-1]
         for i in range(N):
             AL[i], C[i] = qr_left(_ctimes(C[i - 1], A[i]))
         C[-1] = C[-1] / _norm(C[-1])
@@ -196,18 +208,21 @@ def uniform_leftorth(A, C0, tol=1e-14, maxiter=200, eig_miniter=10):
 
 
 def uniform_rightorth(A, C0, tol=1e-14, maxiter=200, eig_miniter=10):
-    """C[i-1] AR[i] = A[i] C[i] (uniform_rightorth!, ortho.jl:274-340)."""
+    """C[i-1] AR[i] = A[i] C[i] (uniform_rightorth!, ortho.jl:274-340).  The
+    error of an iteration is the change of C over the LQ sweep, after the
+    fixed-point step (ortho.jl:308-312)."""
     N = len(A)
     C = [None] * N
     C[-1] = C0 / _norm(C0)
     AR = [None] * N
     eps = math.inf
     for it in range(maxiter):
-        C_before = C[-1]
         if it >= eig_miniter:
-            _, vec = arnoldi_dominant(lambda v: _right_cell(v, A, AR), C[-1], tol=max(eps * eps, 1e-15))
+            _, vec = arnoldi_dominant(lambda v: _right_cell(v, A, AR), C[-1], tol=max(eps * eps, 1e-15),
+                                      krylovdim=30, maxiter=200)
             Qt, Rt = _pos_qr(vec.t())
             C[-1] = Rt.t()
+        C_before = C[-1]
         for i in range(N - 1, -1, -1):
             Lm, AR[i] = lq_right(_timesc(A[i], C[i]))
             C[i - 1] = Lm
@@ -292,23 +307,49 @@ def _chan(G, lo, hi):
     return T.as_colmajor(G[:, lo:hi, :])
 
 
+# inner-cycle target of the refined GMRES, relative to the cycle's starting
+# residual: about the relative error of one FP32 / BF16x3 transfer
+_ETA = {"fp32": 1e-5, "bf16x3": 1e-5}
+
+
+def _fixed_point_solve(apply, b, x0, tol, precision):
+    """GMRES on apply(x, "fp64") x = b; in FP32 / BF16x3 the Krylov steps use
+    apply(x, precision) and every restart recomputes the FP64 residual."""
+    if precision == "fp64":
+        return gmres(lambda x: apply(x, "fp64"), b, x0, tol)
+    return gmres(lambda x: apply(x, "fp64"), b, x0, tol, maxiter=200,
+                 apply_inner=lambda x: apply(x, precision), eta=_ETA[precision])
+
+
+def _cell_ops(prepare, As, Ws, precision):
+    precs = ("fp64",) if precision == "fp64" else ("fp64", precision)
+    return [prepare(A, A, O=W, precisions=precs) for A, W in zip(As, Ws)]
+
+
 class InfiniteEnvironments:
     """GLs[i]: left environment of site i (D x w x D); GRs[i]: right environment of
-    site i (bond between i and i+1).  Solved per channel in Jordan order."""
+    site i (bond between i and i+1).  Solved per channel in Jordan order.
 
-    def __init__(self, psi, H, tol=1e-12):
+    The fixed-point equations of the identity and diagonal channels
+    (infinite_envs.jl:207-346) are solved by GMRES over prepared transfer
+    matrices of the unit cell; with ``precision`` "fp32" or "bf16x3" their
+    Krylov steps run the transfer GEMMs in that precision while every restart
+    takes the FP64 residual, so the solution meets the same FP64 tolerance."""
+
+    def __init__(self, psi, H, tol=1e-12, precision="fp64"):
         self.GLs, self.GRs = None, None
         self.tol = tol
+        self.precision = precision
         self.recalculate(psi, H)
 
     def recalculate(self, psi, H, tol=None):
         tol = self.tol if tol is None else tol
-        self.GLs = self._left(psi, H, tol)
-        self.GRs = self._right(psi, H, tol)
+        self.GLs = self._left(psi, H, tol, self.precision)
+        self.GRs = self._right(psi, H, tol, self.precision)
         return self
 
     @staticmethod
-    def _left(psi, H, tol):
+    def _left(psi, H, tol, precision="fp64"):
         N, w = psi.N, H.w
         D = [psi.AL[i].shape[0] for i in range(N)]
         GL = [T.zeros((D[i], w, D[i])) for i in range(N)]
@@ -331,15 +372,18 @@ class InfiniteEnvironments:
             kind = H.diag_kind(i)
             if kind == "identity":
                 I0 = torch.eye(D[0], dtype=torch.float64, device=b.device)
+                ops = _cell_ops(prepare_transfer_left, psi.AL, [None] * N, precision)
 
-                def apply(x):
+                def apply(x, p):
                     y = x
-                    for s in range(N):
-                        y = transfer_left(y, psi.AL[s], psi.AL[s])
+                    for op in ops:
+                        y = op(y, precision=p)
                     y = y - torch.trace(rho_r @ y) * I0        # regularize!(v, r_LL, l_LL)
                     return x - y
 
-                x, _ = gmres(apply, b, T.as_colmajor(prev), tol)
+                x, _ = _fixed_point_solve(apply, b, T.as_colmajor(prev), tol, precision)
+                for op in ops:
+                    op.free()
                 GL[0][:, i, :] = x
                 if N > 1:
                     cycle(i)
@@ -348,14 +392,18 @@ class InfiniteEnvironments:
                     g = GL[s][:, i, :]
                     GL[s][:, i, :] = g - torch.trace(g @ r) * torch.eye(g.shape[0], dtype=g.dtype, device=g.device)
             elif kind == "diagonal":
-                def apply(x):
+                ops = _cell_ops(prepare_transfer_left, psi.AL,
+                                [T.as_colmajor(H.W[s][i:i + 1, :, :, i:i + 1]) for s in range(N)], precision)
+
+                def apply(x, p):
                     y = x[:, None, :]
-                    for s in range(N):
-                        y = transfer_left(T.as_colmajor(y), T.as_colmajor(H.W[s][i:i + 1, :, :, i:i + 1]),
-                                          psi.AL[s], psi.AL[s])
+                    for op in ops:
+                        y = op(T.as_colmajor(y), precision=p)
                     return x - y[:, 0, :]
 
-                x, _ = gmres(apply, b, T.as_colmajor(prev), tol)
+                x, _ = _fixed_point_solve(apply, b, T.as_colmajor(prev), tol, precision)
+                for op in ops:
+                    op.free()
                 GL[0][:, i, :] = x
                 if N > 1:
                     cycle(i)
@@ -364,7 +412,7 @@ class InfiniteEnvironments:
         return GL
 
     @staticmethod
-    def _right(psi, H, tol):
+    def _right(psi, H, tol, precision="fp64"):
         N, w = psi.N, H.w
         D = [psi.AR[i].shape[2] for i in range(N)]
         GR = [T.zeros((D[i], w, D[i])) for i in range(N)]
@@ -387,15 +435,18 @@ class InfiniteEnvironments:
             kind = H.diag_kind(i)
             if kind == "identity":
                 I0 = torch.eye(D[N - 1], dtype=torch.float64, device=b.device)
+                ops = _cell_ops(prepare_transfer_right, psi.AR[::-1], [None] * N, precision)
 
-                def apply(x):
+                def apply(x, p):
                     y = x
-                    for s in range(N - 1, -1, -1):
-                        y = transfer_right(y, psi.AR[s], psi.AR[s])
+                    for op in ops:
+                        y = op(y, precision=p)
                     y = y - torch.trace(rho_l @ y) * I0        # regularize!(v, l_RR, r_RR)
                     return x - y
 
-                x, _ = gmres(apply, b, T.as_colmajor(prev), tol)
+                x, _ = _fixed_point_solve(apply, b, T.as_colmajor(prev), tol, precision)
+                for op in ops:
+                    op.free()
                 GR[N - 1][:, i, :] = x
                 if N > 1:
                     cycle(i)
@@ -404,14 +455,19 @@ class InfiniteEnvironments:
                     g = GR[s][:, i, :]
                     GR[s][:, i, :] = g - torch.trace(g @ l) * torch.eye(g.shape[0], dtype=g.dtype, device=g.device)
             elif kind == "diagonal":
-                def apply(x):
+                ops = _cell_ops(prepare_transfer_right, psi.AR[::-1],
+                                [T.as_colmajor(H.W[s][i:i + 1, :, :, i:i + 1]) for s in range(N - 1, -1, -1)],
+                                precision)
+
+                def apply(x, p):
                     y = x[:, None, :]
-                    for s in range(N - 1, -1, -1):
-                        y = transfer_right(T.as_colmajor(y), T.as_colmajor(H.W[s][i:i + 1, :, :, i:i + 1]),
-                                           psi.AR[s], psi.AR[s])
+                    for op in ops:
+                        y = op(T.as_colmajor(y), precision=p)
                     return x - y[:, 0, :]
 
-                x, _ = gmres(apply, b, T.as_colmajor(prev), tol)
+                x, _ = _fixed_point_solve(apply, b, T.as_colmajor(prev), tol, precision)
+                for op in ops:
+                    op.free()
                 GR[N - 1][:, i, :] = x
                 if N > 1:
                     cycle(i)
@@ -447,41 +503,58 @@ def calc_galerkin(psi, H, envs):
 
 @dataclass
 class VUMPS:
-    """VUMPS (vumps.jl:23-55); ``precision`` applies to the AC / C eigensolver applies."""
+    """VUMPS (vumps.jl:23-55); ``precision`` applies to the AC / C eigensolver
+    applies and to the transfer GEMMs of the environment fixed-point solves
+    (both refined to their FP64 tolerances).
+
+    The inner solvers take the reference's dynamic tolerances
+    (utility/defaults.jl:23-33, utility/dynamictols.jl:86-89): at iteration
+    ``it`` with Galerkin error eps each uses clamp(factor * eps / sqrt(it),
+    tol_min, tol_max) with factor 1e-3 for the eigensolves, 1e-6 for the
+    gauge fixing and 1e-4 for the environments; the eigensolver is Lanczos
+    with krylovdim 30 and up to 200 restarts."""
     tol: float = 1e-10
     maxiter: int = 100
     verbosity: int = 1
     precision: str = "fp64"
-    tol_factor: float = 0.1
-    tol_min: float = 1e-13
+    tol_factor: float = 1e-3
+    gauge_tol_factor: float = 1e-6
+    envs_tol_factor: float = 1e-4
+    tol_min: float = 1e-14
     tol_max: float = 1e-4
-    krylovdim: int = 20
+    krylovdim: int = 30
+    eig_maxiter: int = 200
+
+    def dynamic_tol(self, factor, it, eps):
+        return min(max(factor * eps / math.sqrt(it), self.tol_min), self.tol_max)
 
 
 def find_groundstate_vumps(psi: InfiniteMPS, H: InfiniteHamiltonian, alg: VUMPS):
     """``find_groundstate(psi::InfiniteMPS, H, VUMPS())`` (vumps.jl:56-125)."""
-    envs = InfiniteEnvironments(psi, H, tol=1e-10)
+    envs = InfiniteEnvironments(psi, H, tol=1e-10, precision=alg.precision)
     eps = calc_galerkin(psi, H, envs)
     history = []
     precs = ("fp64",) if alg.precision == "fp64" else ("fp64", alg.precision)
     for it in range(1, alg.maxiter + 1):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        tol = min(max(alg.tol_factor * eps, alg.tol_min), alg.tol_max)
+        tol = alg.dynamic_tol(alg.tol_factor, it, eps)
         new_AL = []
         for s in range(psi.N):
             # localupdate (_localupdate_vumps_step!, vumps.jl:156-173)
             Hac = prepare_operator(JordanMPO_AC_Hamiltonian(envs.GLs[s], H.W[s], envs.GRs[s]), precisions=precs)
-            _, AC, _ = Hac.eigsolve(psi.AC[s], krylovdim=alg.krylovdim, maxiter=20, tol=tol, precision=alg.precision)
+            _, AC, _ = Hac.eigsolve(psi.AC[s], krylovdim=alg.krylovdim, maxiter=alg.eig_maxiter, tol=tol,
+                                    precision=alg.precision)
             Hac.free()
             Hc = prepare_operator(MPO_C_Hamiltonian(envs.GLs[(s + 1) % psi.N], envs.GRs[s]), precisions=precs)
-            _, C, _ = Hc.eigsolve(psi.C[s], krylovdim=alg.krylovdim, maxiter=20, tol=tol, precision=alg.precision)
+            _, C, _ = Hc.eigsolve(psi.C[s], krylovdim=alg.krylovdim, maxiter=alg.eig_maxiter, tol=tol,
+                                  precision=alg.precision)
             Hc.free()
             new_AL.append(regauge(AC, C))
         # gauge_step! (vumps.jl:175-183): right-orthogonalise the new AL, AC = AL C
-        AR, C = uniform_rightorth(new_AL, psi.C[-1], tol=max(tol, 1e-14))
+        AR, C = uniform_rightorth(new_AL, psi.C[-1], tol=alg.dynamic_tol(alg.gauge_tol_factor, it, eps))
         psi = InfiniteMPS(new_AL, AR, C)
-        envs.recalculate(psi, H, tol=max(tol * 0.1, 1e-14))
+        envs.recalculate(psi, H, tol=alg.dynamic_tol(alg.envs_tol_factor, it, eps))
         eps = calc_galerkin(psi, H, envs)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0

@@ -401,5 +401,7 @@ def jordan_ac2_apply_nn(ICm, BEm, CB, x):
     y += BEm @ m
     y = y.reshape(Dl, d1, Dr, d2, order="F")
     if CB is not None:
-        y += _es("ahcj,behj->abce", x, CB)
+        # y[a,b,c,e] += sum_{h,j} x[a,h,c,j] CB[b,e,h,j], as one BLAS product
+        # over the d^2 physical pairs
+        y += np.tensordot(x, CB, axes=([1, 3], [2, 3])).transpose(0, 2, 1, 3)
     return y

@@ -102,7 +102,7 @@ def check_all(mk, op, x, y_ref, cols=None, label=""):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("D", [512, 1024, 2048, 2056, 4096])
-def test_jordan_ac2_config_sizes(mk, D):
+def test_jordan_ac2_bench_sizes(mk, D):
     """The headline apply (Jordan two-site, nearest neighbour) against the full oracle."""
     W, GL, GR = inputs(D)
     x = unit(4, (D, 2, D, 2))
@@ -116,7 +116,7 @@ def test_jordan_ac2_config_sizes(mk, D):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("D", [512, 2048, 4096])
-def test_jordan_ac_config_sizes(mk, D):
+def test_jordan_ac_bench_sizes(mk, D):
     """Single-site Jordan derivative at the same sizes (full oracle)."""
     W, GL, GR = inputs(D)
     x = unit(4, (D, 2, D))
@@ -129,7 +129,7 @@ def test_jordan_ac_config_sizes(mk, D):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("D", [512, 1024, 2048, 4096])
-def test_generic_ac2_config_sizes(mk, D):
+def test_generic_ac2_bench_sizes(mk, D):
     """Generic (precomputed) w = 5 two-site derivative; D = 4096 through 48 sampled
     output columns (exact restriction of the oracle)."""
     W, GL, GR = inputs(D)
@@ -145,7 +145,7 @@ def test_generic_ac2_config_sizes(mk, D):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("D", [2048, 4096])
-def test_generic_ac_config_sizes(mk, D):
+def test_generic_ac_bench_sizes(mk, D):
     """Generic w = 5 single-site derivative; D = 4096 through sampled output columns."""
     W, GL, GR = inputs(D)
     x = unit(4, (D, 2, D))

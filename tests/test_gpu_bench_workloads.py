@@ -47,7 +47,7 @@ def sector_dims(k, q):
     return SECTORS[i] if 0 <= i < len(SECTORS) else 0
 
 
-def test_u1_sector_profile_is_the_config():
+def test_u1_sector_profile_matches_the_bench():
     assert sum(SECTORS) == 2048 and len(SECTORS) == 21
 
 

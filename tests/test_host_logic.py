@@ -144,18 +144,45 @@ def test_truncated_split_without_truncation_is_exact(direction, shape):
         assert float(torch.linalg.matrix_norm(iso.t() @ iso - torch.eye(k, dtype=torch.float64))) < 1e-12
 
 
+def test_rationalize_is_the_smallest_denominator_within_tolerance():
+    """rationalize(x, tol) against a brute-force search over denominators, and
+    the values Julia's rationalize gives for pi (22//7 at tol 0.01)."""
+    from fractions import Fraction
+    from mpskit_amd.observables import rationalize
+    assert rationalize(np.pi, 0.01) == Fraction(22, 7)
+    assert rationalize(-0.97, 0.1) == -1 and rationalize(0.3 / np.pi, 0.1) == 0
+    rng = np.random.default_rng(5)
+    for x, tol in zip(rng.uniform(-1, 1, 200), 10.0 ** rng.uniform(-4, -0.5, 200)):
+        r = rationalize(x, tol)
+        assert abs(Fraction(float(x)) - r) <= Fraction(float(tol))
+        for q in range(1, r.denominator):
+            p = round(x * q)
+            assert all(abs(Fraction(float(x)) - Fraction(pp, q)) > Fraction(float(tol)) for pp in (p - 1, p, p + 1))
+
+
 def test_marek_gap_and_correlation_length_of_a_synthetic_spectrum():
-    """marek_gap drops the modulus-1 eigenvalue, groups by angle (in units of pi)
-    and returns -log|lambda_2| and the gap to the next eigenvalue at that angle."""
-    from mpskit_amd.observables import marek_gap
+    """marek_gap (transfer_spectrum.jl:136-160) drops the modulus-1 eigenvalue,
+    snaps the angles to the simplest fraction of pi within tol_angle and groups
+    by the snapped angle.  Angles 0.3, -0.3 and 0.31 (0.096, -0.096 and 0.099 in
+    units of pi) all snap to 0 at tol 0.1, so theta = 0 and the first two
+    -log|lambda| coincide (delta = 0); -2 snaps to -2/3 pi."""
+    from mpskit_amd.observables import approx_angles, marek_gap
+    from fractions import Fraction
     lam2 = 0.6 * np.exp(0.3j)
-    # 0.55 e^{-2i} sits at another angle; the conjugate pair of lambda_2 at -0.3
-    # differs by 0.19 pi from 0.3, also another group
     spec = np.array([1.0, lam2, np.conj(lam2), 0.5 * np.exp(0.31j), 0.55 * np.exp(-2.0j)])
+    assert approx_angles(spec[1:], 0.1) == [0, 0, 0, Fraction(-2, 3)]
     eps, delta, theta = marek_gap(spec, tol_angle=0.1)
     assert abs(eps + np.log(0.6)) < 1e-14
-    assert abs(theta - 0.3) < 1e-14
-    assert abs(delta - (np.log(0.6) - np.log(0.5))) < 1e-14
+    assert theta == 0.0 and delta == 0.0
+    # finer tolerance: 0.3 / pi = 0.0955 snaps to 1/10 with tol 0.01; only
+    # lambda_2 and 0.5 e^{0.31 i} (0.0987) share it: two eigenvalues, delta = inf
+    eps, delta, theta = marek_gap(spec, tol_angle=0.01)
+    assert abs(theta - np.pi / 10) < 1e-15 and delta == np.inf and abs(eps + np.log(0.6)) < 1e-14
+    # branch cut: angles near -pi snap to -1, mapped to +1
+    spec = np.array([1.0, 0.7 * np.exp(-0.97j * np.pi), 0.6 * np.exp(0.98j * np.pi), 0.4 * np.exp(0.99j * np.pi)])
+    eps, delta, theta = marek_gap(spec, tol_angle=0.1)
+    assert theta == np.pi and abs(eps + np.log(0.7)) < 1e-14
+    assert abs(delta - (np.log(0.7) - np.log(0.6))) < 1e-14
 
 
 def test_mpo_product_is_the_matrix_square():
