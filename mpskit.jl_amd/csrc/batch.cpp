@@ -9,9 +9,11 @@
 // grouped operand conversion, and one grouped persistent GEMM over all
 // instances' x*IC + BE*x products.  Other operators are applied one by one.
 //
-// The batched eigensolver runs the restarted Lanczos / FP64-refinement scheme
-// of mk_eigsolve for every instance in lockstep: one grouped apply per Krylov
-// step, segmented float64 Gram-Schmidt, and one device->host readback per step.
+// The batched eigensolver runs the thick-restart Lanczos / FP64-refinement
+// scheme of mk_eigsolve for every instance in lockstep: one grouped apply per
+// Krylov step, segmented float64 Gram-Schmidt, and one device->host readback per
+// step.  Every instance keeps the same number of Ritz vectors at a restart (the
+// fewest any active instance has), so the basis layout is common.
 // Optionally the device work of Krylov step j is captured once into a hipGraph
 // and replayed (all buffers and descriptor arrays it touches are owned by the
 // batch and keep their addresses).
@@ -29,6 +31,8 @@
 namespace mk {
 
 void lanczos_lowest(const std::vector<double>& G, int ld, int k, double& lam, std::vector<double>& s);
+void lanczos_lowest_p(const std::vector<double>& G, int ld, int k, int p, std::vector<double>& lam,
+                      std::vector<double>& S);
 
 namespace {
 
@@ -62,7 +66,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
   // eigensolver state (grown on demand; graphs refer to these addresses)
-  int ks = 0;
+  int ks = 0, kq = 0;      // Krylov columns of Q, and the Ritz-vector columns after them
   DevBuf Q, W, U, coef, partial;
   double* host = nullptr;  // pinned readback / upload buffer
   size_t host_doubles = 0;
@@ -393,17 +397,19 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
   MK_CHECK_HIP(hipSetDevice(ctx->device));
   MK_TRY(b->ensure_plan(MK_FP64));
   MK_TRY(b->ensure_plan(precision));
-  const int n = b->n, m = krylovdim - 1, ks = m + 2;
+  const int n = b->n, ks = krylovdim + 1;
+  const int pmax = std::max(1, krylovdim / 2);   // Ritz vectors kept at a restart (x included)
   const int64_t N = b->total;
   const bool graphs = use_graph && b->grouped;
   // buffers (addresses are part of captured graphs: reallocation drops them)
-  if (b->ks < ks || !b->Q.ptr) {
+  if (b->ks < ks || b->kq < pmax || !b->Q.ptr) {
     MK_TRY(b->wait_streams());
     b->drop_graphs();
     b->Q.release();
     b->coef.release();
     b->partial.release();
-    MK_TRY(b->Q.alloc((size_t)N * ks * 8));
+    MK_TRY(b->Q.alloc((size_t)N * (ks + pmax) * 8));
+    b->kq = pmax;
     MK_TRY(b->coef.alloc((size_t)n * (3 * ks + 8) * 8));
     MK_TRY(b->partial.alloc((size_t)ks * b->sg.nchunks * 8));
     if (!b->W.ptr) MK_TRY(b->W.alloc((size_t)N * 8));
@@ -435,7 +441,7 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
   double* dv_b = dv_a + n;
   double* dv_s = dv_b + n;
   double* dv_t = dv_s + (int64_t)n * kst;  // scalar results (nseg)
-  MK_CHECK_HIP(hipMemsetAsync(Qp, 0, (size_t)N * kst * 8, st));
+  MK_CHECK_HIP(hipMemsetAsync(Qp, 0, (size_t)N * (kst + b->kq) * 8, st));
   MK_CHECK_HIP(hipMemsetAsync(Wp, 0, (size_t)N * 8, st));
   MK_CHECK_HIP(hipMemsetAsync(Up, 0, (size_t)N * 8, st));
   for (int i = 0; i < n; ++i)
@@ -462,12 +468,14 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
   }
   MK_TRY(seg_normalize(sg, col(0), dv_t, 1, col(0), st));
 
-  std::vector<double> theta(n, 0.0), res(n, INFINITY), est_last(n, INFINITY), thresh(n, 0.0);
+  std::vector<double> theta(n, 0.0), res(n, INFINITY), est_last(n, INFINITY), thresh(n, 0.0), rperp(n, 0.0);
   std::vector<int> conv(n, 0), ops_n(n, 0), active(n, 0), kk(n, 0);
-  const int ldg = m + 2;
+  const int ldg = ks;
   std::vector<std::vector<double>> G(n, std::vector<double>((size_t)ldg * ldg));
+  std::vector<std::vector<double>> lamk(n, std::vector<double>(1, 0.0)), cx(n);
   const double eta = precision == MK_FP64 ? 0.0 : 1e-3;
-  std::vector<double> ones(n, 1.0), zeros(n, 0.0), neg(n), sv;
+  std::vector<double> ones(n, 1.0), zeros(n, 0.0), neg(n), sv, lamv, Sv;
+  int p = 1;   // kept vectors [x, y_2 .. y_p] in columns 0 .. p-1 of every segment
   for (int outer = 0;; ++outer) {
     if (precision == MK_FP64 && outer > 0)
       for (int i = 0; i < n; ++i)
@@ -478,23 +486,25 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
     bool all = true;
     for (int i = 0; i < n; ++i) all = all && conv[i];
     if (all) break;
-    // float64 Ritz values and residuals
+    // float64 Ritz values and residuals; <q_r, H x> of the kept vectors
     MK_TRY(b->apply(col(0), Wp, MK_FP64, st));
     for (int i = 0; i < n; ++i)
       if (!conv[i]) ++ops_n[i];
-    MK_TRY(seg_dots(sg, col(0), N, 1, Wp, dv_t, 1, part, st));
-    MK_TRY(readback(dv_t, n));
+    MK_TRY(seg_dots(sg, Qp, N, p, Wp, c, kst, part, st));
+    MK_TRY(readback(c, n * kst));
     for (int i = 0; i < n; ++i) {
-      if (!conv[i]) theta[i] = hb[i];
-      neg[i] = -hb[i];
+      cx[i].assign(hb + (int64_t)i * kst, hb + (int64_t)i * kst + p);
+      if (!conv[i]) theta[i] = cx[i][0];
+      neg[i] = -cx[i][0];
     }
-    MK_CHECK_HIP(hipMemcpyAsync(col(1), Wp, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+    // residual r = W - theta x in column p, re-orthogonalised against x
+    MK_CHECK_HIP(hipMemcpyAsync(col(p), Wp, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
     MK_TRY(upload(dv_a, neg));
     MK_TRY(upload(dv_b, ones));
-    MK_TRY(seg_axpby(sg, dv_a, col(0), dv_b, col(1), st));
-    MK_TRY(seg_dots(sg, col(0), N, 1, col(1), dv_t, 1, part, st));
-    MK_TRY(seg_multiaxpy(sg, col(0), N, 1, dv_t, 1, col(1), st));
-    MK_TRY(seg_dots(sg, col(1), N, 1, col(1), dv_t, 1, part, st));
+    MK_TRY(seg_axpby(sg, dv_a, col(0), dv_b, col(p), st));
+    MK_TRY(seg_dots(sg, col(0), N, 1, col(p), dv_t, 1, part, st));
+    MK_TRY(seg_multiaxpy(sg, col(0), N, 1, dv_t, 1, col(p), st));
+    MK_TRY(seg_dots(sg, col(p), N, 1, col(p), dv_t, 1, part, st));
     MK_TRY(readback(dv_t, n));
     bool any = false;
     for (int i = 0; i < n; ++i) {
@@ -506,24 +516,66 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
       else any = true;
     }
     if (!any || outer == maxiter) break;
-    MK_TRY(seg_normalize(sg, col(1), dv_t, 1, col(1), st));
+    // the Krylov space starts from the part of r outside the kept Ritz vectors
+    for (int i = 0; i < n; ++i) rperp[i] = res[i];
+    if (p > 1) {
+      for (int pass = 0; pass < 2; ++pass) {
+        MK_TRY(seg_dots(sg, Qp, N, p, col(p), c, kst, part, st));
+        MK_TRY(seg_multiaxpy(sg, Qp, N, p, c, kst, col(p), st));
+      }
+      MK_TRY(seg_dots(sg, col(p), N, 1, col(p), dv_t, 1, part, st));
+      MK_TRY(readback(dv_t, n));
+      bool lost = false;
+      for (int i = 0; i < n; ++i) {
+        if (conv[i]) continue;
+        rperp[i] = std::sqrt(std::max(hb[i], 0.0));
+        lost = lost || !(rperp[i] > 1e-14 * std::max(std::fabs(theta[i]), res[i]));
+      }
+      if (lost) {
+        // some residual lies in the kept space: restart every instance from x alone
+        p = 1;
+        for (int i = 0; i < n; ++i) {
+          rperp[i] = res[i];
+          cx[i].resize(1);
+        }
+        MK_CHECK_HIP(hipMemcpyAsync(col(1), Wp, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+        MK_TRY(upload(dv_a, neg));
+        MK_TRY(seg_axpby(sg, dv_a, col(0), dv_b, col(1), st));
+        MK_TRY(seg_dots(sg, col(0), N, 1, col(1), dv_t, 1, part, st));
+        MK_TRY(seg_multiaxpy(sg, col(0), N, 1, dv_t, 1, col(1), st));
+      }
+    }
+    {
+      std::vector<double> sq(n);
+      for (int i = 0; i < n; ++i) sq[i] = conv[i] ? 1.0 : rperp[i] * rperp[i];
+      MK_TRY(upload(dv_t, sq));
+    }
+    MK_TRY(seg_normalize(sg, col(p), dv_t, 1, col(p), st));
     for (int i = 0; i < n; ++i) {
-      std::fill(G[i].begin(), G[i].end(), 0.0);
-      G[i][0] = theta[i];
-      G[i][1] = G[i][ldg] = res[i];
+      std::vector<double>& Gi = G[i];
+      std::fill(Gi.begin(), Gi.end(), 0.0);
+      Gi[0] = theta[i];
+      for (int r = 1; r < p; ++r) {
+        Gi[(size_t)r * ldg + r] = lamk[i][r];
+        Gi[r] = Gi[(size_t)r * ldg] = cx[i][r];
+      }
+      Gi[p] = Gi[(size_t)p * ldg] = rperp[i];
       kk[i] = 1;
       active[i] = conv[i] ? 0 : 1;
       thresh[i] = std::max(tol, eta * res[i]);
     }
-    for (int j = 0; j < m; ++j) {
+    // Krylov step j applies column p + j and writes column p + j + 1 (at most ks - 1)
+    const int mmax = ks - p - 1;
+    for (int j = 0; j < mmax; ++j) {
+      const int jj = p + j - 1;   // krylov_step(jj) applies column jj + 1
       if (graphs) {
-        const auto key = std::make_pair(precision, j);
+        const auto key = std::make_pair(precision, jj);
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
           hipGraph_t g = nullptr;
           hipGraphExec_t ex = nullptr;
           MK_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-          const mk_status s = b->krylov_step(j, precision, st);
+          const mk_status s = b->krylov_step(jj, precision, st);
           const hipError_t e = hipStreamEndCapture(st, &g);
           MK_TRY(s);
           MK_CHECK_HIP(e);
@@ -534,53 +586,66 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
         }
         MK_CHECK_HIP(hipGraphLaunch(it->second, st));
       } else {
-        MK_TRY(b->krylov_step(j, precision, st));
+        MK_TRY(b->krylov_step(jj, precision, st));
       }
       MK_CHECK_HIP(hipStreamSynchronize(st));
       const double* c1 = hb;
       const double* c2 = hb + (int64_t)n * kst;
       const double* nn = hb + 2 * (int64_t)n * kst;
       const double* xw = nn + n;
+      const int cj = p + j;
       bool still = false;
       for (int i = 0; i < n; ++i) {
         if (!active[i]) continue;
         ++ops_n[i];
         std::vector<double>& Gi = G[i];
-        for (int r = 1; r < j + 2; ++r) Gi[(size_t)r * ldg + (j + 1)] = c1[(int64_t)i * kst + r] + c2[(int64_t)i * kst + r];
+        for (int r = 1; r <= cj; ++r) Gi[(size_t)r * ldg + cj] = Gi[(size_t)cj * ldg + r] = c1[(int64_t)i * kst + r] + c2[(int64_t)i * kst + r];
         const double beta = std::sqrt(std::max(nn[i], 0.0));
         MK_REQUIRE(std::isfinite(beta), MK_NONFINITE, "batch eigsolve: non-finite Krylov vector %d", i);
         double lam;
-        lanczos_lowest(Gi, ldg, j + 2, lam, sv);
-        est_last[i] = beta * std::fabs(sv[j + 1]);
+        lanczos_lowest(Gi, ldg, cj + 1, lam, sv);
+        est_last[i] = beta * std::fabs(sv[cj]);
         kk[i] = j + 1;
-        if (est_last[i] <= thresh[i] || beta <= 1e-14 * std::fabs(theta[i]) || j + 1 == m) {
+        if (est_last[i] <= thresh[i] || beta <= 1e-14 * std::fabs(theta[i]) || j + 1 == mmax) {
           active[i] = 0;
           continue;
         }
-        Gi[(size_t)(j + 2) * ldg + (j + 1)] = beta;
-        Gi[(size_t)(j + 2)] = Gi[(size_t)(j + 2) * ldg] = xw[i];
+        Gi[(size_t)(cj + 1) * ldg + cj] = Gi[(size_t)cj * ldg + cj + 1] = beta;
+        Gi[(size_t)(cj + 1)] = Gi[(size_t)(cj + 1) * ldg] = xw[i];
         still = true;
       }
       if (!still) break;
     }
-    // lowest Ritz vectors become the new x (converged instances keep theirs)
-    std::vector<double> S((size_t)n * kst, 0.0);
-    int kmax = 1;
+    // the lowest Ritz vectors of every instance's space: x and the kept vectors
+    // of the next cycle (converged instances keep their columns)
+    int pn = pmax, kmax = 1;
+    for (int i = 0; i < n; ++i)
+      if (!conv[i]) pn = std::min(pn, p + kk[i]);
+    std::vector<std::vector<double>> Si(n);
     for (int i = 0; i < n; ++i) {
-      if (conv[i]) {
-        S[(size_t)i * kst] = 1.0;
-        continue;
-      }
-      double lam;
-      lanczos_lowest(G[i], ldg, kk[i] + 1, lam, sv);
-      for (int r = 0; r <= kk[i]; ++r) S[(size_t)i * kst + r] = sv[r];
-      theta[i] = lam;
-      kmax = std::max(kmax, kk[i] + 1);
+      if (conv[i]) continue;
+      lanczos_lowest_p(G[i], ldg, p + kk[i], pn, lamv, Si[i]);
+      lamk[i] = lamv;
+      theta[i] = lamv[0];
+      kmax = std::max(kmax, p + kk[i]);
     }
-    MK_TRY(upload(dv_s, S));
-    MK_TRY(seg_gemv(sg, Qp, N, kmax, dv_s, kst, Up, st));
-    MK_TRY(seg_dots(sg, Up, N, 1, Up, dv_t, 1, part, st));
-    MK_TRY(seg_normalize(sg, Up, dv_t, 1, col(0), st));
+    for (int q = 0; q < pn; ++q) {
+      std::vector<double> S((size_t)n * kst, 0.0);
+      for (int i = 0; i < n; ++i) {
+        if (conv[i]) {
+          S[(size_t)i * kst + q] = 1.0;
+          continue;
+        }
+        const int k = p + kk[i];
+        for (int r = 0; r < k; ++r) S[(size_t)i * kst + r] = Si[i][(size_t)q * k + r];
+      }
+      MK_TRY(upload(dv_s, S));
+      MK_TRY(seg_gemv(sg, Qp, N, std::max(kmax, pn), dv_s, kst, col(ks + q), st));
+    }
+    MK_CHECK_HIP(hipMemcpyAsync(col(0), col(ks), (size_t)N * pn * 8, hipMemcpyDeviceToDevice, st));
+    MK_TRY(seg_dots(sg, col(0), N, 1, col(0), dv_t, 1, part, st));
+    MK_TRY(seg_normalize(sg, col(0), dv_t, 1, col(0), st));
+    p = pn;
   }
   for (int i = 0; i < n; ++i)
     MK_CHECK_HIP(hipMemcpyAsync(x + b->off[i], col(0) + b->off[i], (size_t)b->len[i] * 8, hipMemcpyDeviceToDevice, st));

@@ -29,53 +29,10 @@
 #include "gemm.h"
 #include "kernels.h"
 #include "ops.h"
+#include "symeig.h"
 
 namespace mk {
 namespace {
-
-// Cyclic Jacobi eigen-decomposition of a small dense symmetric matrix
-// (row-major n x n).  Returns eigenvalues in `ev` and eigenvectors as columns
-// of `Q` (row-major).
-void jacobi_eig(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& Q) {
-  Q.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int p = 0; p < n; ++p)
-      for (int q = 0; q < n; ++q) {
-        const double v = A[(size_t)p * n + q] * A[(size_t)p * n + q];
-        tot += v;
-        if (p != q) off += v;
-      }
-    if (off <= 1e-34 * tot || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = A[(size_t)p * n + q];
-        if (std::fabs(apq) < 1e-300) continue;
-        const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
-          A[(size_t)k * n + p] = c * akp - s * akq;
-          A[(size_t)k * n + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
-          A[(size_t)p * n + k] = c * apk - s * aqk;
-          A[(size_t)q * n + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double qkp = Q[(size_t)k * n + p], qkq = Q[(size_t)k * n + q];
-          Q[(size_t)k * n + p] = c * qkp - s * qkq;
-          Q[(size_t)k * n + q] = s * qkp + c * qkq;
-        }
-      }
-  }
-  ev.resize(n);
-  for (int i = 0; i < n; ++i) ev[i] = A[(size_t)i * n + i];
-}
 
 // the p lowest eigenpairs of the leading k x k block of G (row-major, stride
 // ld): eigenvalues ascending in lam, eigenvectors as columns of S (k x p,
@@ -86,15 +43,12 @@ void lowest_p(const std::vector<double>& G, int ld, int k, int p, std::vector<do
   for (int i = 0; i < k; ++i)
     for (int j = 0; j < k; ++j) A[(size_t)i * k + j] = 0.5 * (G[(size_t)i * ld + j] + G[(size_t)j * ld + i]);
   std::vector<double> ev, Q;
-  jacobi_eig(A, k, ev, Q);
-  std::vector<int> idx(k);
-  for (int i = 0; i < k; ++i) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+  sym_eig(std::move(A), k, ev, Q);   // ascending
   lam.resize(p);
   S.resize((size_t)k * p);
   for (int c = 0; c < p; ++c) {
-    lam[c] = ev[idx[c]];
-    for (int i = 0; i < k; ++i) S[(size_t)c * k + i] = Q[(size_t)i * k + idx[c]];
+    lam[c] = ev[c];
+    for (int i = 0; i < k; ++i) S[(size_t)c * k + i] = Q[(size_t)i * k + c];
   }
 }
 
@@ -104,13 +58,10 @@ void lowest(const std::vector<double>& G, int ld, int k, double& lam, std::vecto
   for (int i = 0; i < k; ++i)
     for (int j = 0; j < k; ++j) A[(size_t)i * k + j] = 0.5 * (G[(size_t)i * ld + j] + G[(size_t)j * ld + i]);
   std::vector<double> ev, Q;
-  jacobi_eig(A, k, ev, Q);
-  int imin = 0;
-  for (int i = 1; i < k; ++i)
-    if (ev[i] < ev[imin]) imin = i;
-  lam = ev[imin];
+  sym_eig(std::move(A), k, ev, Q);   // ascending
+  lam = ev[0];
   s.resize(k);
-  for (int i = 0; i < k; ++i) s[i] = Q[(size_t)i * k + imin];
+  for (int i = 0; i < k; ++i) s[i] = Q[(size_t)i * k];
 }
 
 struct Dev {
