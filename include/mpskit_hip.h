@@ -225,6 +225,21 @@ mk_status mk_transfer_right_mpo_levels(mk_ctx ctx, const mk_env_levels* v, const
 mk_status mk_op_set_charge_blocks(mk_op op, int nl, const int64_t* loff, const int* lq, int nr,
                                   const int64_t* roff, const int* rq, const int* pq);
 
+/* Compact storage of the charge blocks (TensorKit's block-sparse storage, one
+ * dense block per charge sector, src/utility/utility.jl:283-303): after
+ * mk_op_set_charge_blocks, x and y of every apply hold only the blocks that
+ * conserve charge, block b column-major (m_b x n_b) at element offset off_b
+ * (mk_op_charge_blocks lists them), so the Krylov vectors, the physical term
+ * and the mixed-precision conversions touch the blocks only.  mk_op_info then
+ * reports the compact sizes.  No counterpart call in the reference: its tensors
+ * are block-sparse from the start. */
+mk_status mk_op_compact_charge_blocks(mk_op op);
+/* The blocks of a charge-blocked operator: 5 entries per block (dense row
+ * offset r0 and rows m of the fused (Dl d1) side, column offset c0 and columns
+ * n of the fused (Dr d2) side, compact element offset), up to `cap` blocks;
+ * *nblocks receives the count.  out may be NULL to query the count. */
+mk_status mk_op_charge_blocks(mk_op op, int64_t* out, int cap, int* nblocks);
+
 /* ---- derivative operators (apply) ---------------------------------------- */
 
 /* y = alpha * H(x) + beta * y using arithmetic `precision` (which must have

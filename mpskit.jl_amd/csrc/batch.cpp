@@ -100,7 +100,7 @@ namespace {
 bool nn_jordan(const Op* op) {
   // charge-blocked operators leave y outside their blocks to the physical pass
   return op->kind == OP_AC2_JOR && !op->CA.present() && op->AB.empty() && !op->gen.present() &&
-         (op->IC.present() || op->BE.present()) && (op->blocks.empty() || op->hasP);
+         (op->IC.present() || op->BE.present()) && (op->blocks.empty() || op->hasP) && !op->compact;
 }
 
 }  // namespace

@@ -17,11 +17,13 @@
 // into AB when AB exists, and CB folds into AB when only AB exists.
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "group.h"
 #include "kernels.h"
 #include "ops.h"
+#include "seg.h"
 
 namespace mk {
 
@@ -289,7 +291,146 @@ mk_status apply_blocks(Op* op, const double* x, double* y, double alpha, double 
   return MK_OK;
 }
 
+// GEMM problems of the charge blocks on compact storage: x at slot 1, y at
+// slot 2, the mixed views of x at slot 3 (per block: the left-factor view
+// [m][pad8(n)], then the right-factor view [n][pad8(m)]; bf16 hi then lo).
+struct CompactViews {
+  std::vector<int64_t> a, b, alo, blo;   // byte offsets in slot 3
+  int64_t bytes = 0;
+};
+CompactViews compact_views(const Op* op, int prec) {
+  CompactViews v;
+  if (prec == MK_FP64) return v;
+  const int64_t eb = prec == MK_BF16X3 ? 2 : 4;
+  const bool bf = prec == MK_BF16X3;
+  int64_t o = 0;
+  for (const Op::BlockProb& bp : op->blocks) {
+    const int64_t na = bp.m * pad8(bp.n) * eb, nb = bp.n * pad8(bp.m) * eb;
+    v.a.push_back(o); o += align256(na);
+    v.alo.push_back(bf ? o : 0); if (bf) o += align256(na);
+    v.b.push_back(o); o += align256(nb);
+    v.blo.push_back(bf ? o : 0); if (bf) o += align256(nb);
+  }
+  v.bytes = o;
+  return v;
+}
+
+int compact_block(const Op* op, int64_t r0, int64_t c0) {
+  for (size_t b = 0; b < op->blocks.size(); ++b)
+    if (op->blocks[b].r0 == r0 && op->blocks[b].c0 == c0) return (int)b;
+  return -1;
+}
+
+void compact_probs(const Op* op, int prec, std::vector<ProbSpec>& out) {
+  const int64_t M = op->Dl * op->d1, N = op->Dr * op->d2;
+  const bool bf = prec == MK_BF16X3;
+  const int64_t eb = bf ? 2 : 4;
+  const CompactViews cv = compact_views(op, prec);
+  for (size_t b = 0; b < op->blocks.size(); ++b) {
+    const Op::BlockProb& bp = op->blocks[b];
+    ProbSpec p;
+    p.M = bp.m;
+    p.N = bp.n;
+    p.C = const_cast<double*>(static_cast<const double*>(rel_ptr(2, op->boff[b] * 8)));
+    p.ldc = bp.m;
+    p.alpha = 1.0;
+    p.beta = 1.0;
+    for (int q = 0; q < bp.nseg; ++q) {
+      const Op::BlockSeg& sg = bp.seg[q];
+      const int xb = sg.kind == 'I' ? compact_block(op, bp.r0, sg.k0) : compact_block(op, sg.k0, bp.c0);
+      if (xb < 0) continue;   // x has no block there: the product is zero
+      if (prec == MK_FP64) {
+        if (sg.kind == 'I')
+          p.add(rel_ptr(1, op->boff[xb] * 8), bp.m, op->IC.f64() + sg.k0 + bp.c0 * N, N, sg.k);
+        else
+          p.add(op->BE.f64() + bp.r0 + sg.k0 * M, M, rel_ptr(1, op->boff[xb] * 8), sg.k, sg.k);
+        continue;
+      }
+      if (sg.kind == 'I') {
+        const OpMat& m = op->IC;
+        const char* hi = static_cast<const char*>(bf ? m.hi.ptr : m.f32.ptr) + (bp.c0 * m.kpad + sg.k0) * eb;
+        const char* lo = bf ? static_cast<const char*>(m.lo.ptr) + (bp.c0 * m.kpad + sg.k0) * eb : nullptr;
+        p.add(rel_ptr(3, cv.a[xb]), pad8(sg.k), hi, m.kpad, sg.k, bf ? rel_ptr(3, cv.alo[xb]) : nullptr, lo);
+      } else {
+        const OpMat& m = op->BE;
+        const char* hi = static_cast<const char*>(bf ? m.hi.ptr : m.f32.ptr) + (bp.r0 * m.kpad + sg.k0) * eb;
+        const char* lo = bf ? static_cast<const char*>(m.lo.ptr) + (bp.r0 * m.kpad + sg.k0) * eb : nullptr;
+        p.add(hi, m.kpad, rel_ptr(3, cv.b[xb]), pad8(sg.k), sg.k, lo, bf ? rel_ptr(3, cv.blo[xb]) : nullptr);
+      }
+    }
+    out.push_back(p);
+  }
+}
+
+// Charge-block apply on compact storage: the physical term (every output
+// block a combination of the blocks with its bond sectors), the mixed views
+// of every block, then one grouped GEMM over the blocks' products.
+mk_status apply_compact(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  Ctx* ctx = op->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = op->x_elems;
+  MK_REQUIRE(prec == MK_FP64 || op->blocks_aligned, MK_NOT_SUPPORTED,
+             "charge-block apply: mixed precision needs sector ranges in multiples of 8");
+  if (prec != MK_FP64 && !op->csplit[prec].ptr) {
+    // the conversion items of the mixed views, made once per precision
+    const CompactViews cv = compact_views(op, prec);
+    std::vector<SplitItem> items;
+    int mx = 0;
+    for (size_t b = 0; b < op->blocks.size(); ++b) {
+      const Op::BlockProb& bp = op->blocks[b];
+      for (int role = 0; role < 2; ++role) {
+        SplitItem it{};
+        it.src = static_cast<const double*>(rel_ptr(1, op->boff[b] * 8));
+        it.rows = bp.m;
+        it.cols = bp.n;
+        it.ld = bp.m;
+        it.trans = role;   // 0: left-factor view (K = columns), 1: right-factor view (K = rows)
+        it.ldo = role == 0 ? pad8(bp.n) : pad8(bp.m);
+        it.nblocks = split_blocks(it.rows, it.cols, it.trans, it.ldo);
+        it.hi = const_cast<void*>(rel_ptr(3, role == 0 ? cv.a[b] : cv.b[b]));
+        it.lo = prec == MK_BF16X3 ? const_cast<void*>(rel_ptr(3, role == 0 ? cv.alo[b] : cv.blo[b])) : nullptr;
+        mx = std::max(mx, (int)it.nblocks);
+        items.push_back(it);
+      }
+    }
+    MK_TRY(op->csplit[prec].alloc(items.size() * sizeof(SplitItem)));
+    MK_CHECK_HIP(hipMemcpyAsync(op->csplit[prec].ptr, items.data(), items.size() * sizeof(SplitItem),
+                                hipMemcpyHostToDevice, st));
+    MK_CHECK_HIP(hipStreamSynchronize(st));
+    op->ncsplit[prec] = (int)items.size();
+    op->csplit_max[prec] = mx;
+    op->cview[prec] = cv.bytes;
+  }
+  const size_t vbytes = prec == MK_FP64 ? 0 : (size_t)op->cview[prec];
+  void *mv = nullptr, *tmp = nullptr;
+  MK_TRY(carve(ctx, {{vbytes, &mv}, {alpha != 1.0 ? (size_t)n * 8 : 0, &tmp}}));
+  double* out = alpha != 1.0 ? static_cast<double*>(tmp) : y;
+  const double b0 = alpha != 1.0 ? 0.0 : beta;
+  const void* base[NBASE] = {nullptr, x, out, mv};
+  if (op->hasP)
+    MK_TRY(cphys_group(static_cast<const CPhysItem*>(op->cphys.ptr), op->ncphys, op->cphys_max, 1.0, b0, base, st));
+  else
+    MK_TRY(k_axpby(0.0, out, b0, out, n, st));
+  if (prec != MK_FP64)
+    MK_TRY(split_group(static_cast<const SplitItem*>(op->csplit[prec].ptr), op->ncsplit[prec], op->csplit_max[prec],
+                       prec == MK_BF16X3, base, st));
+  auto* plan = static_cast<GroupPlan*>(op->bplan[prec].get());
+  if (!plan) {
+    std::vector<ProbSpec> probs;
+    compact_probs(op, prec, probs);
+    auto gp = std::make_shared<GroupPlan>();
+    const int64_t bb[NBASE] = {0, n * 8, n * 8, (int64_t)vbytes};
+    MK_TRY(group_plan(ctx, probs.data(), (int)probs.size(), prec, GemmShape{}, bb, -1, gp.get()));
+    op->bplan[prec] = gp;
+    plan = gp.get();
+  }
+  MK_TRY(plan->run(base, st));
+  if (alpha != 1.0) MK_TRY(k_axpby(alpha, out, beta, y, n, st));
+  return MK_OK;
+}
+
 mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, double beta, int prec) {
+  if (op->compact) return apply_compact(op, x, y, alpha, beta, prec);
   if (!op->blocks.empty()) return apply_blocks(op, x, y, alpha, beta, prec);
   Ctx* ctx = op->ctx;
   const int64_t Dl = op->Dl, Dr = op->Dr, d1 = op->d1, d2 = op->d2;
@@ -620,6 +761,84 @@ extern "C" mk_status mk_op_set_charge_blocks(mk_op h, int nl, const int64_t* lof
   op->blocks_aligned = aligned;
   for (auto& b : op->bplan) b.reset();
   op->flops = fl + (op->hasP ? 2.0 * (double)op->x_elems * op->d1 * op->d2 : 0.0);
+  return MK_OK;
+}
+
+extern "C" mk_status mk_op_compact_charge_blocks(mk_op h) {
+  MK_REQUIRE(h != nullptr, MK_BAD_ARG, "compact charge blocks: null operator");
+  Op* op = reinterpret_cast<Op*>(h);
+  MK_REQUIRE(!op->blocks.empty(), MK_BAD_ARG, "compact charge blocks: set the charge blocks first");
+  MK_REQUIRE(!op->compact, MK_BAD_ARG, "compact charge blocks: already compact");
+  MK_CHECK_HIP(hipSetDevice(op->ctx->device));
+  hipStream_t st = op->ctx->stream;
+  const int64_t Dl = op->Dl, Dr = op->Dr, d1 = op->d1, d2 = op->d2;
+  std::vector<int64_t> off;
+  int64_t nnz = 0;
+  for (const Op::BlockProb& bp : op->blocks) {
+    off.push_back(nnz);
+    nnz += bp.m * bp.n;
+  }
+  std::vector<CPhysItem> items;
+  int mx = 0;
+  if (op->hasP) {
+    std::vector<double> P((size_t)d1 * d2 * d1 * d2);
+    MK_CHECK_HIP(hipMemcpyAsync(P.data(), op->P.ptr, P.size() * 8, hipMemcpyDeviceToHost, st));
+    MK_CHECK_HIP(hipStreamSynchronize(st));
+    std::map<std::pair<int64_t, int64_t>, int> at;
+    for (size_t b = 0; b < op->blocks.size(); ++b) at[{op->blocks[b].r0, op->blocks[b].c0}] = (int)b;
+    for (size_t b = 0; b < op->blocks.size(); ++b) {
+      const Op::BlockProb& bp = op->blocks[b];
+      // y[a,pb,c,pe] += sum P[pb,pe,h,j] x[a,h,c,j]: the same bond sectors at other physical indices
+      const int64_t pb = bp.r0 / Dl, lo = bp.r0 % Dl, pe = bp.c0 / Dr, ro = bp.c0 % Dr;
+      CPhysItem it{};
+      it.y = static_cast<double*>(const_cast<void*>(rel_ptr(2, off[b] * 8)));
+      it.len = bp.m * bp.n;
+      it.nblocks = cphys_blocks(it.len);
+      for (int64_t j = 0; j < d2; ++j)
+        for (int64_t hh = 0; hh < d1; ++hh) {
+          const double c = P[pb + d1 * (pe + d2 * (hh + d1 * j))];
+          auto f = at.find({lo + Dl * hh, ro + Dr * j});
+          if (c == 0.0 || f == at.end()) continue;
+          const Op::BlockProb& sb = op->blocks[f->second];
+          MK_REQUIRE(sb.m == bp.m && sb.n == bp.n, MK_BAD_SHAPE, "compact charge blocks: inconsistent sectors");
+          MK_REQUIRE(it.nsrc < CPHYS_MAX, MK_NOT_SUPPORTED, "compact charge blocks: d1 d2 > %d", CPHYS_MAX);
+          it.src[it.nsrc] = static_cast<const double*>(rel_ptr(1, off[f->second] * 8));
+          it.coef[it.nsrc] = c;
+          ++it.nsrc;
+        }
+      mx = std::max(mx, (int)it.nblocks);
+      items.push_back(it);
+    }
+    MK_TRY(op->cphys.alloc(items.size() * sizeof(CPhysItem)));
+    MK_CHECK_HIP(hipMemcpyAsync(op->cphys.ptr, items.data(), items.size() * sizeof(CPhysItem), hipMemcpyHostToDevice,
+                                st));
+    MK_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  op->ncphys = (int)items.size();
+  op->cphys_max = mx;
+  op->boff = std::move(off);
+  op->compact = true;
+  op->x_elems = op->y_elems = nnz;
+  for (auto& b : op->bplan) b.reset();
+  return MK_OK;
+}
+
+extern "C" mk_status mk_op_charge_blocks(mk_op h, int64_t* out, int cap, int* nblocks) {
+  MK_REQUIRE(h != nullptr && nblocks != nullptr, MK_BAD_ARG, "charge blocks: null argument");
+  Op* op = reinterpret_cast<Op*>(h);
+  *nblocks = (int)op->blocks.size();
+  int64_t o = 0;
+  for (size_t b = 0; b < op->blocks.size() && (int)b < cap; ++b) {
+    const Op::BlockProb& bp = op->blocks[b];
+    if (out) {
+      out[5 * b + 0] = bp.r0;
+      out[5 * b + 1] = bp.m;
+      out[5 * b + 2] = bp.c0;
+      out[5 * b + 3] = bp.n;
+      out[5 * b + 4] = o;
+    }
+    o += bp.m * bp.n;
+  }
   return MK_OK;
 }
 

@@ -134,6 +134,16 @@ struct Op {
   std::vector<BlockProb> blocks;
   bool blocks_aligned = false;       // every range a multiple of 8 (mixed precisions allowed)
   std::shared_ptr<void> bplan[3];    // GroupPlan per precision, built on first use
+  // compact storage of the charge blocks: x and y hold only the blocks, block b
+  // column-major (m x n) at element offset boff[b]; the grouped items of the
+  // physical term (device) and of the mixed views
+  bool compact = false;
+  std::vector<int64_t> boff;
+  DevBuf cphys;
+  int ncphys = 0, cphys_max = 0;
+  DevBuf csplit[3];
+  int ncsplit[3] = {0, 0, 0}, csplit_max[3] = {0, 0, 0};
+  int64_t cview[3] = {0, 0, 0};      // bytes of the mixed views per precision
 };
 
 // Grouped-GEMM problems of an operator's charge blocks: x at slot 1 + xoff, y at

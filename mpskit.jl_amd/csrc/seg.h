@@ -37,6 +37,19 @@ struct PhysItem {
   double alpha, beta;
 };
 
+// One output block of the two-site physical term on charge-block storage:
+// y = alpha sum_s coef[s] src[s] + beta y over `len` contiguous elements (the
+// source blocks hold the same bond sectors as y, so the sum is elementwise);
+// y and src may be relative.
+constexpr int CPHYS_MAX = 16;
+struct CPhysItem {
+  double* y;
+  const double* src[CPHYS_MAX];
+  double coef[CPHYS_MAX];
+  int64_t len;
+  int32_t nsrc, nblocks;
+};
+
 // out[s * ostride + j] = <V_j, u> restricted to segment s, j < k.
 // `partial` holds k * nchunks doubles.
 mk_status seg_dots(const Segs& sg, const double* V, int64_t ldv, int k, const double* u, double* out, int ostride,
@@ -58,6 +71,9 @@ int split_blocks(int64_t rows, int64_t cols, int trans, int64_t ldo);
 mk_status phys2_group(const PhysItem* items, int n, int max_blocks, int d12, const void* const base[NBASE],
                       hipStream_t st);
 int phys2_blocks(int64_t Dl, int64_t Dr);
+mk_status cphys_group(const CPhysItem* items, int n, int max_blocks, double alpha, double beta,
+                      const void* const base[NBASE], hipStream_t st);
+int cphys_blocks(int64_t len);
 
 // One nearest-neighbour two-site prologue (d1 = d2 = 2): y = alpha CB(x) + beta y
 // and the K-contiguous conversions of x as left factor (xa, row length ka) and

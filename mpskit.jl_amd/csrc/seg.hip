@@ -216,6 +216,32 @@ __global__ void phys2_group_kernel(const PhysItem* __restrict__ items, BaseArgs 
   }
 }
 
+// Physical term on charge-block storage, one output block per blockIdx.y,
+// 4 elements per thread.
+__global__ void cphys_group_kernel(const CPhysItem* __restrict__ items, double alpha, double beta, BaseArgs ba) {
+  const CPhysItem& d = items[blockIdx.y];
+  const int64_t i0 = ((int64_t)blockIdx.x * TPB + threadIdx.x) * 4;
+  if (i0 >= d.len) return;
+  typedef __attribute__((address_space(1))) double gdouble;
+  gdouble* y = (gdouble*)rslv(ba, d.y);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nsrc = d.nsrc;
+  for (int s = 0; s < nsrc; ++s) {
+    const gdouble* x = (const gdouble*)rslv(ba, d.src[s]);
+    const double c = d.coef[s];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (i0 + t < d.len) acc[t] += c * x[i0 + t];
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (i0 + t >= d.len) break;
+    double v = alpha * acc[t];
+    if (beta != 0.0) v += beta * y[i0 + t];
+    y[i0 + t] = v;
+  }
+}
+
 template <bool BF>
 __global__ __launch_bounds__(256) void nn_group_kernel(const NNItem* __restrict__ items, BaseArgs ba) {
   __shared__ ProTile t;
@@ -309,6 +335,17 @@ mk_status phys2_group(const PhysItem* items, int n, int max_blocks, int d12, con
 }
 
 int phys2_blocks(int64_t Dl, int64_t Dr) { return (int)cdiv(Dl * Dr, (int64_t)TPB); }
+
+mk_status cphys_group(const CPhysItem* items, int n, int max_blocks, double alpha, double beta,
+                      const void* const base[NBASE], hipStream_t st) {
+  if (n == 0 || max_blocks == 0) return MK_OK;
+  MK_REQUIRE(n < 65536, MK_BAD_ARG, "cphys_group: too many items");
+  hipLaunchKernelGGL(cphys_group_kernel, dim3(max_blocks, n), dim3(TPB), 0, st, items, alpha, beta, to_args(base));
+  MK_CHECK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+int cphys_blocks(int64_t len) { return (int)cdiv(len, (int64_t)TPB * 4); }
 
 void nn_tiles(int64_t Dl, int64_t Dr, int32_t& tiles_a, int32_t& tiles) {
   tiles_a = (int32_t)cdiv(Dl, (int64_t)PRO_A);

@@ -95,6 +95,8 @@ _SIGS = {
                                      [C.POINTER(EnvLevels)], _INT),
     "mk_op_set_charge_blocks": ([_P, _INT, C.POINTER(_I64), C.POINTER(_INT), _INT, C.POINTER(_I64), C.POINTER(_INT),
                                  C.POINTER(_INT)], _INT),
+    "mk_op_compact_charge_blocks": ([_P], _INT),
+    "mk_op_charge_blocks": ([_P, C.POINTER(_I64), _INT, C.POINTER(_INT)], _INT),
     "mk_apply": ([_P, _P, _P, _D, _D, _INT], _INT),
     "mk_op_info": ([_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_D)], _INT),
     "mk_op_free": ([_P], _INT),

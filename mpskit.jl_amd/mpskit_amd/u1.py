@@ -76,6 +76,47 @@ def set_charge_blocks(op, left: U1Bond, right: U1Bond, phys_q=PHYS_Q):
     return op
 
 
+class CompactLayout:
+    """Compact charge-block storage of a charge-blocked two-site operator
+    (``mk_op_compact_charge_blocks``): ``gather`` takes a dense column-major
+    (Dl, d1, Dr, d2) tensor to the compact vector of its charge blocks,
+    ``scatter`` puts a compact vector back into a dense tensor (zero outside the
+    blocks)."""
+
+    def __init__(self, op, shape):
+        nb = C.c_int()
+        check(load().mk_op_charge_blocks(op.handle, None, 0, C.byref(nb)))
+        buf = (C.c_int64 * (5 * nb.value))()
+        check(load().mk_op_charge_blocks(op.handle, buf, nb.value, C.byref(nb)))
+        blocks = np.frombuffer(buf, dtype=np.int64).reshape(nb.value, 5)
+        Dl, d1, Dr, d2 = shape
+        M = Dl * d1
+        idx = []
+        for r0, m, c0, n, _ in blocks:
+            idx.append((np.arange(r0, r0 + m)[:, None] + M * np.arange(c0, c0 + n)[None, :]).ravel(order="F"))
+        self.blocks = blocks
+        self.shape = tuple(shape)
+        self.idx = torch.from_numpy(np.concatenate(idx)).to(T.device())
+        self.n = int(self.idx.numel())
+
+    def gather(self, x):
+        return T.as_colmajor(x).permute(3, 2, 1, 0).reshape(-1)[self.idx].contiguous()
+
+    def scatter(self, xc):
+        out = T.zeros(self.shape)
+        out.permute(3, 2, 1, 0).reshape(-1)[self.idx] = xc.reshape(-1)
+        return out
+
+
+def compact_charge_blocks(op, shape):
+    """Switch a charge-blocked operator to compact storage; returns its layout
+    (the operator's x and y become the compact vectors)."""
+    check(load().mk_op_compact_charge_blocks(op.handle))
+    lay = CompactLayout(op, shape)
+    op.x_shape = op.y_shape = (lay.n,)
+    return lay
+
+
 def _block_rows(left: U1Bond, phys_q, c):
     """Fused (l, p) indices of middle charge c, l fastest (column-major fusion)."""
     idx = []
@@ -294,10 +335,15 @@ def _aligned(b: U1Bond):
 
 
 def find_groundstate_u1(psi: U1FiniteMPS, H, trunc_dim, sweeps=2, precision="fp64", tol=1e-10, krylovdim=20,
-                        maxiter=50, use_graph=True, phys_q=PHYS_Q, verbosity=0):
+                        maxiter=50, use_graph=True, phys_q=PHYS_Q, verbosity=0, storage="compact"):
     """U(1) DMRG2 sweeps (dmrg.jl:286-354) on the charge-block path.  The center
     starts at site 0 (as built by U1FiniteMPS.random).  Returns history
-    (per sweep: energy, time, solve time, applies, bond dims at the middle)."""
+    (per sweep: energy, time, solve time, applies, bond dims at the middle).
+
+    ``storage`` "compact" solves every local problem on the compact vector of
+    its charge blocks (``mk_op_compact_charge_blocks``); "dense" keeps the dense
+    charge-sorted two-site vector and solves through the batch path
+    (hipGraph-captured Krylov steps)."""
     L = psi.L
     envs = FiniteEnvironments(psi, H)
     precs = ("fp64",) if precision == "fp64" else ("fp64", precision)
@@ -315,16 +361,27 @@ def find_groundstate_u1(psi: U1FiniteMPS, H, trunc_dim, sweeps=2, precision="fp6
             set_charge_blocks(op, lb, rb, phys_q)
             prec = precision if (_aligned(lb) and _aligned(rb)) else "fp64"
             x0 = psi.AC2(pos)
-            Bt = BatchedDerivative([op])
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            _, vecs, infos = Bt.eigsolve([x0], krylovdim=krylovdim, maxiter=maxiter, tol=tol, precision=prec,
-                                         use_graph=use_graph)
-            torch.cuda.synchronize()
-            ts += time.perf_counter() - t1
-            nops += infos[0]["numops"]
-            u1_gauge2(psi, pos, direction, vecs[0], trunc_dim, 0.0, phys_q)
-            Bt.free()
+            if storage == "compact":
+                lay = compact_charge_blocks(op, x0.shape)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                _, xc, info = op.eigsolve(lay.gather(x0), krylovdim=krylovdim, maxiter=maxiter, tol=tol,
+                                          precision=prec)
+                torch.cuda.synchronize()
+                ts += time.perf_counter() - t1
+                nops += info["numops"]
+                u1_gauge2(psi, pos, direction, lay.scatter(xc), trunc_dim, 0.0, phys_q)
+            else:
+                Bt = BatchedDerivative([op])
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                _, vecs, infos = Bt.eigsolve([x0], krylovdim=krylovdim, maxiter=maxiter, tol=tol, precision=prec,
+                                             use_graph=use_graph)
+                torch.cuda.synchronize()
+                ts += time.perf_counter() - t1
+                nops += infos[0]["numops"]
+                u1_gauge2(psi, pos, direction, vecs[0], trunc_dim, 0.0, phys_q)
+                Bt.free()
             op.free()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0

