@@ -209,6 +209,16 @@ mk_status mk_transfer_right_mpo_levels(mk_ctx ctx, const mk_env_levels* v, const
                                        const double* A, const double* Abar, int64_t Da, int64_t Di,
                                        int64_t Dc, int64_t Do, const mk_env_levels* y);
 
+/* Two classical Gram-Schmidt passes of u against the k <= 32 orthonormal
+ * columns of V (column-major, leading dim ldv; device pointers): u becomes
+ * u - V (c1 + c2), h[0..k) = c1 + c2 and h[k] = |u| (host array of k + 1).
+ * The orthogonalisation step of the Krylov solvers around the library
+ * (KrylovKit's ClassicalGramSchmidt2 under the Arnoldi / GMRES / Lanczos calls
+ * of infinite_envs.jl:207-346, ortho.jl:307-313, transfer_spectrum.jl:27-60,
+ * tdvp.jl:72-133); no counterpart entry point in the reference. */
+mk_status mk_krylov_orthogonalize(mk_ctx ctx, const double* V, int64_t ldv, int k, double* u, int64_t n,
+                                  double* h);
+
 /* ---- U(1) charge blocks ------------------------------------------------------
  * Counterpart of TensorKit's block-sparse U(1) storage under the same
  * derivative operators (the reference applies one GEMM per fusion-tree block,
@@ -291,7 +301,7 @@ mk_status mk_transfer_right_mpo(mk_ctx ctx, const double* v, const double* O,
                                 int64_t Dc, int64_t Do, double* y);
 
 /* A transfer matrix prepared for repeated applies (a TransferMatrix of the
- * reference, src/transfermatrix/transfermatrix.jl:11-33, applied by the
+ * reference, src/transfermatrix/transfermatrix.jl:4-33, applied by the
  * fixed-point solvers of src/environments/infinite_envs.jl:207-346 and
  * src/states/ortho.jl:307-313): mk_apply(op, x, y, alpha, beta, precision)
  * computes y = alpha T(x) + beta y in any prepared precision, mk_eigsolve and

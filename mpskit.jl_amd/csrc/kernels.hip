@@ -456,36 +456,42 @@ __global__ void reduce_rows_kernel(const double* __restrict__ partial, int nb, d
 // w^T u (MODE 1, 2 outputs).  Each thread keeps its row of V in registers
 // between the update and the dot products, so V is read once for both.
 // k <= CGS_KMAX; partial sums go to partial[o * gridDim.x + blockIdx.x].
+// KT (4, 8, 16 or 32 >= k) sizes the register arrays: the rows of V held per
+// thread, and the accumulators, scale with the basis, so small bases keep
+// the occupancy (and the memory parallelism) high.  MODE 2: no update, the
+// partial sums of V^T u only (the first Gram-Schmidt pass).
 constexpr int CGS_KMAX = 32;
-template <int MODE>
-__global__ __launch_bounds__(TPB) void cgs_update_kernel(const double* __restrict__ V, int64_t ldv, int k,
+template <int MODE, int KT, typename VT>
+__global__ __launch_bounds__(TPB) void cgs_update_kernel(const VT* __restrict__ V, int64_t ldv, int k,
                                                          const double* __restrict__ cin, double* __restrict__ u,
                                                          const double* __restrict__ w, int64_t n,
                                                          double* __restrict__ partial) {
   __shared__ double sh[TPB / 64];
-  constexpr int NO = MODE == 0 ? CGS_KMAX : 2;
-  double c[CGS_KMAX], acc[NO];
+  constexpr int NO = MODE == 1 ? 2 : KT;
+  double c[KT], acc[NO];
 #pragma unroll
-  for (int j = 0; j < CGS_KMAX; ++j) c[j] = j < k ? cin[j] : 0.0;
+  for (int j = 0; j < KT; ++j) c[j] = (MODE != 2 && j < k) ? cin[j] : 0.0;
 #pragma unroll
   for (int o = 0; o < NO; ++o) acc[o] = 0.0;
   GRID_STRIDE(r, n) {
-    double q[CGS_KMAX];
+    double q[KT];
 #pragma unroll
-    for (int j = 0; j < CGS_KMAX; ++j) q[j] = j < k ? V[r + (int64_t)j * ldv] : 0.0;
+    for (int j = 0; j < KT; ++j) q[j] = j < k ? (double)V[r + (int64_t)j * ldv] : 0.0;
     double v = u[r];
+    if constexpr (MODE != 2) {
 #pragma unroll
-    for (int j = 0; j < CGS_KMAX; ++j) v -= c[j] * q[j];
-    u[r] = v;
-    if constexpr (MODE == 0) {
+      for (int j = 0; j < KT; ++j) v -= c[j] * q[j];
+      u[r] = v;
+    }
+    if constexpr (MODE != 1) {
 #pragma unroll
-      for (int j = 0; j < CGS_KMAX; ++j) acc[j] += q[j] * v;
+      for (int j = 0; j < KT; ++j) acc[j] += q[j] * v;
     } else {
       acc[0] += v * v;
       acc[1] += w[r] * v;
     }
   }
-  const int nout = MODE == 0 ? k : 2;
+  const int nout = MODE == 1 ? 2 : k;
 #pragma unroll
   for (int o = 0; o < NO; ++o) {
     if (o >= nout) break;
@@ -796,9 +802,38 @@ mk_status k_regularize(double* v, const double* lvec, const double* rvec, int64_
 
 int multidot_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, (int64_t)TPB * MD_ITEMS)); }
 
+int cgs_blocks(int64_t n);
+
+template <int MODE, typename VT>
+void launch_cgs(int k, int nb, hipStream_t st, const VT* V, int64_t ldv, const double* cin, double* u,
+                const double* w, int64_t n, double* partial) {
+  if (k <= 4)
+    hipLaunchKernelGGL((cgs_update_kernel<MODE, 4, VT>), dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, w, n,
+                       partial);
+  else if (k <= 8)
+    hipLaunchKernelGGL((cgs_update_kernel<MODE, 8, VT>), dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, w, n,
+                       partial);
+  else if (k <= 16)
+    hipLaunchKernelGGL((cgs_update_kernel<MODE, 16, VT>), dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, w, n,
+                       partial);
+  else
+    hipLaunchKernelGGL((cgs_update_kernel<MODE, 32, VT>), dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, w, n,
+                       partial);
+}
+
 mk_status k_multidot(const double* V, int64_t ldv, int k, const double* w, int64_t n, double* out,
                      double* partial, hipStream_t st) {
   if (k <= 0) return MK_OK;
+  if (k <= CGS_KMAX) {
+    // one read of V with register accumulators; at most multidot_blocks(n)
+    // blocks, so callers' partial buffers sized for the wide kernel suffice
+    const int nb = std::min(cgs_blocks(n), multidot_blocks(n));
+    launch_cgs<2, double>(k, nb, st, V, ldv, nullptr, const_cast<double*>(w), nullptr, n, partial);
+    MK_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(k), dim3(TPB), 0, st, partial, nb, out);
+    MK_CHECK_HIP(hipGetLastError());
+    return MK_OK;
+  }
   const int nb = multidot_blocks(n);
   hipLaunchKernelGGL(multidot_kernel, dim3(nb), dim3(TPB), 0, st, V, ldv, k, w, n, partial);
   MK_CHECK_HIP(hipGetLastError());
@@ -812,7 +847,7 @@ mk_status k_cgs_update_dot(const double* V, int64_t ldv, int k, const double* ci
                            double* out, double* partial, hipStream_t st) {
   MK_REQUIRE(k >= 1 && k <= CGS_KMAX, MK_BAD_ARG, "cgs_update_dot: k = %d not in [1, %d]", k, CGS_KMAX);
   const int nb = cgs_blocks(n);
-  hipLaunchKernelGGL(cgs_update_kernel<0>, dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, nullptr, n, partial);
+  launch_cgs<0, double>(k, nb, st, V, ldv, cin, u, nullptr, n, partial);
   MK_CHECK_HIP(hipGetLastError());
   hipLaunchKernelGGL(reduce_rows_kernel, dim3(k), dim3(TPB), 0, st, partial, nb, out);
   MK_CHECK_HIP(hipGetLastError());
@@ -823,7 +858,7 @@ mk_status k_cgs_update_norm(const double* V, int64_t ldv, int k, const double* c
                             int64_t n, double* out, double* partial, hipStream_t st) {
   MK_REQUIRE(k >= 1 && k <= CGS_KMAX, MK_BAD_ARG, "cgs_update_norm: k = %d not in [1, %d]", k, CGS_KMAX);
   const int nb = cgs_blocks(n);
-  hipLaunchKernelGGL(cgs_update_kernel<1>, dim3(nb), dim3(TPB), 0, st, V, ldv, k, cin, u, w, n, partial);
+  launch_cgs<1, double>(k, nb, st, V, ldv, cin, u, w, n, partial);
   MK_CHECK_HIP(hipGetLastError());
   hipLaunchKernelGGL(reduce_rows_kernel, dim3(2), dim3(TPB), 0, st, partial, nb, out);
   MK_CHECK_HIP(hipGetLastError());
@@ -839,6 +874,7 @@ mk_status k_gemv(const double* V, int64_t ldv, int k, const double* s, double* x
                  hipStream_t st) {
   LAUNCH_1D(gemv_kernel, n, st, V, ldv, k, s, x, n);
 }
+
 mk_status k_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t st) {
   LAUNCH_1D(axpby_kernel, n, st, a, x, b, y, n);
 }

@@ -116,6 +116,37 @@ static double inner_eta(int precision) {
 
 using namespace mk;
 
+// Two classical Gram-Schmidt passes of u against the k <= 32 columns of V
+// (KrylovKit's ClassicalGramSchmidt2, the orthogonalisation of the Arnoldi,
+// Lanczos and GMRES solvers of the reference): u <- u - V (c1 + c2) with
+// c1 = V^T u and c2 = V^T (u - V c1), then |u|.  h[0..k) = c1 + c2 and
+// h[k] = |u| on the host.  Three reads of V, one host sync.
+extern "C" mk_status mk_krylov_orthogonalize(mk_ctx h, const double* V, int64_t ldv, int k, double* u, int64_t n,
+                                             double* hout) {
+  MK_REQUIRE(h != nullptr && V != nullptr && u != nullptr && hout != nullptr, MK_BAD_ARG,
+             "krylov_orthogonalize: null argument");
+  MK_REQUIRE(k >= 1 && k <= 32, MK_BAD_ARG, "krylov_orthogonalize: k = %d not in [1, 32]", k);
+  MK_REQUIRE(n >= 1 && ldv >= n, MK_BAD_SHAPE, "krylov_orthogonalize: n = %lld, ldv = %lld", (long long)n,
+             (long long)ldv);
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  MK_CHECK_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  void* mem = nullptr;
+  const size_t np = (size_t)std::max<int64_t>((int64_t)cgs_blocks(n) * 34, std::max<int64_t>(1, cdiv(n, 256 * 16)) * 34);
+  MK_TRY(ctx->small((3 * 32 + 2 + np) * 8, &mem));
+  double* cd = static_cast<double*>(mem);
+  double* part = cd + 3 * 32 + 2;
+  MK_TRY(k_multidot(V, ldv, k, u, n, cd, part, st));
+  MK_TRY(k_cgs_update_dot(V, ldv, k, cd, u, n, cd + 32, part, st));
+  MK_TRY(k_cgs_update_norm(V, ldv, k, cd + 32, u, u, n, cd + 64, part, st));
+  std::vector<double> hv(66);
+  MK_CHECK_HIP(hipMemcpyAsync(hv.data(), cd, hv.size() * 8, hipMemcpyDeviceToHost, st));
+  MK_CHECK_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < k; ++i) hout[i] = hv[i] + hv[32 + i];
+  hout[k] = std::sqrt(std::max(hv[64], 0.0));
+  return MK_OK;
+}
+
 extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter, double tol, int precision,
                                  double* eigval, double* resnorm, int* numops, int* converged) {
   MK_REQUIRE(h != nullptr && x != nullptr, MK_BAD_ARG, "eigsolve: null argument");
@@ -130,7 +161,10 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
   hipStream_t st = ctx->stream;
   const int64_t n = op->x_elems;
   const int K = krylovdim;                        // basis columns: x, kept Ritz vectors, Krylov vectors
-  const int pmax = std::max(1, krylovdim / 2);    // Ritz vectors kept at a restart (x included)
+  // Ritz vectors kept at a restart (x included).  A mixed-precision cycle costs
+  // one float64 apply besides its low-precision Krylov steps, so it keeps fewer
+  // (krylovdim / 4) and runs longer cycles.
+  const int pmax = std::max(1, precision == MK_FP64 ? krylovdim / 2 : krylovdim / 4);
 
   // basis Q = [x, y_2..y_p, v_0, ...] (column-major n x K), then pmax columns for
   // the new Ritz vectors; w = H x, u = work

@@ -26,6 +26,7 @@ import numpy as np
 import torch
 
 from . import tensors as T
+from .krylov import cgs2
 from .algorithms import expectation_value, gauge2
 from .derivatives import JordanMPO_AC2_Hamiltonian, prepare_operator
 from .environments import FiniteEnvironments
@@ -50,12 +51,7 @@ def lanczos_min(apply, x0, tol=1e-12, krylovdim=30, maxiter=50):
         for j in range(kmax):
             w = _flat(apply(_unflat(Vb[j], shape)))
             ops += 1
-            Vj = Vb[: j + 1]
-            h1 = Vj @ w
-            w = w - Vj.t() @ h1
-            h2 = Vj @ w
-            w = w - Vj.t() @ h2
-            hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+            hcol = cgs2(Vb, j + 1, w)
             H[: j + 1, j] = hcol[:-1]
             H[j + 1, j] = hcol[-1]
             Tk = H[: j + 1, : j + 1]

@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import tensors as T
+from .krylov import cgs2
 from .states import FiniteMPS, lq_right
 from .transfer import transfer_left
 
@@ -174,12 +175,7 @@ def transfer_spectrum(psi, howmany=20, krylovdim=None, seed=0):
         for s in range(psi.N):
             y = transfer_left(T.as_colmajor(y), psi.AL[s], psi.AL[s])
         w = _flat(y)
-        Vj = Vb[: j + 1]
-        h1 = Vj @ w
-        w = w - Vj.t() @ h1
-        h2 = Vj @ w
-        w = w - Vj.t() @ h2
-        hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+        hcol = cgs2(Vb, j + 1, w)
         H[: j + 1, j] = hcol[:-1]
         H[j + 1, j] = hcol[-1]
         if hcol[-1] <= 1e-14:

@@ -24,6 +24,7 @@ import numpy as np
 import torch
 
 from . import tensors as T
+from .krylov import cgs2
 from .algorithms import truncated_split
 from .derivatives import JordanMPO_AC2_Hamiltonian, JordanMPO_AC_Hamiltonian, MPO_C_Hamiltonian, prepare_operator
 from .environments import FiniteEnvironments
@@ -62,12 +63,7 @@ def exponentiate(op, tau, x0, tol=1e-12, krylovdim=30, maxsteps=256):
         for j in range(kmax):
             w = _flat(op(_unflat(Vb[j], shape)))
             ops += 1
-            Vj = Vb[: j + 1]
-            h1 = Vj @ w
-            w = w - Vj.t() @ h1
-            h2 = Vj @ w
-            w = w - Vj.t() @ h2
-            hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+            hcol = cgs2(Vb, j + 1, w)
             H[: j + 1, j] = hcol[:-1]
             H[j + 1, j] = hcol[-1]
             Tk = H[: j + 1, : j + 1]

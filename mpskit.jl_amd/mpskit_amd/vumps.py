@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import tensors as T
+from .krylov import cgs2
 from .derivatives import JordanMPO_AC_Hamiltonian, MPO_AC_Hamiltonian, MPO_C_Hamiltonian, prepare_operator
 from .states import lq_right, pos_qr, qr_left
 from .transfer import prepare_transfer_left, prepare_transfer_right, transfer_left, transfer_right
@@ -83,12 +84,7 @@ def gmres(apply, b, x0, tol, krylovdim=30, maxiter=50, apply_inner=None, eta=0.0
         for j in range(krylovdim):
             w = _flat(inner(_unflat(Vb[j], shape)))
             ops += 1
-            Vj = Vb[: j + 1]
-            h1 = Vj @ w
-            w = w - Vj.t() @ h1
-            h2 = Vj @ w
-            w = w - Vj.t() @ h2
-            hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+            hcol = cgs2(Vb, j + 1, w)
             H[: j + 1, j] = hcol[:-1]
             H[j + 1, j] = hcol[-1]
             e1 = np.zeros(j + 2)
@@ -134,12 +130,7 @@ def arnoldi_dominant(apply, v0, tol=1e-12, krylovdim=20, maxiter=20, relative=Fa
         for j in range(krylovdim):
             w = _flat(apply(_unflat(Vb[j], shape)))
             ops += 1
-            Vj = Vb[: j + 1]
-            h1 = Vj @ w
-            w = w - Vj.t() @ h1
-            h2 = Vj @ w
-            w = w - Vj.t() @ h2
-            hcol = torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).cpu().numpy()
+            hcol = cgs2(Vb, j + 1, w)
             H[: j + 1, j] = hcol[:-1]
             H[j + 1, j] = hcol[-1]
             ev, evec = np.linalg.eig(H[: j + 1, : j + 1])

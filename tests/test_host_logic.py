@@ -233,11 +233,27 @@ def test_argument_checks_before_any_device_work():
         Ob.dot(SimpleNamespace(L=4), SimpleNamespace(L=5))
 
 
+def _cgs2_gemv(Vb, k, w):
+    """The two Gram-Schmidt passes of mpskit_amd.krylov.cgs2 as plain torch
+    GEMVs (the library kernels behind cgs2 need the GPU; GPU tests check them)."""
+    import torch
+    Vj = Vb[:k]
+    h1 = Vj @ w
+    w -= Vj.t() @ h1
+    h2 = Vj @ w
+    w -= Vj.t() @ h2
+    return torch.cat([h1 + h2, torch.linalg.vector_norm(w)[None]]).numpy()
+
+
 @pytest.fixture
 def cpu_layout(monkeypatch):
-    """The Krylov solvers' vector layout helper keeps CPU tensors on the CPU here."""
+    """The Krylov solvers' vector layout helper keeps CPU tensors on the CPU
+    here, and their orthogonalisation step runs as torch GEMVs."""
+    from mpskit_amd import excited, observables, tdvp, vumps
     from mpskit_amd import tensors as T
     monkeypatch.setattr(T, "as_colmajor", lambda t: t)
+    for mod in (vumps, excited, tdvp, observables):
+        monkeypatch.setattr(mod, "cgs2", _cgs2_gemv)
     return T
 
 
