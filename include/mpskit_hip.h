@@ -258,6 +258,25 @@ mk_status mk_op_charge_blocks(mk_op op, int64_t* out, int cap, int* nblocks);
 mk_status mk_apply(mk_op op, const double* x, double* y, double alpha, double beta,
                    int precision);
 
+/* Allocation-free variant of mk_apply.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * operators and results as mk_apply, but the scratch of the apply (the
+ * FP32 / BF16X3 operand views of x, the intermediate of a generic operator)
+ * comes from a fixed device array of MK_NOALLOC_SCRATCH_MB MiB that the
+ * library holds on every GPU it runs on until the process ends (its address
+ * is resolved at mk_ctx_create); the call allocates nothing, creates no
+ * events and does not synchronise.  An apply whose scratch exceeds the array
+ * returns MK_BAD_SHAPE (160 MiB hold the BF16X3 views of a two-site Jordan
+ * operator up to d*D = 4096, i.e. D = 2048 at d = 2; FP64 applies of Jordan
+ * operators need none), and operators with charge blocks return
+ * MK_NOT_SUPPORTED.  All calls on one GPU share the array: calls that overlap
+ * on one GPU (different streams or threads) race; calls ordered on one stream
+ * do not.  Nothing in the array needs to start at zero. */
+#define MK_NOALLOC_SCRATCH_MB 160
+mk_status mk_apply_noalloc(mk_op op, const double* x, double* y, double alpha, double beta,
+                           int precision);
+
 /* number of float64 elements of x and y, and the algorithmic flop count of one
  * apply (each contraction counted once, whatever the precision scheme) */
 mk_status mk_op_info(mk_op op, int64_t* x_elems, int64_t* y_elems, double* flops);

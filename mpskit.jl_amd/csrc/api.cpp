@@ -54,6 +54,8 @@ mk_status Ctx::scratch(size_t nbytes, void** out) {
 }
 
 mk_status Ctx::small(size_t nbytes, void** out) {
+  MK_REQUIRE(fixed_scratch().base == nullptr, MK_NOT_SUPPORTED,
+             "this call needs the context's own scratch and cannot run without allocating");
   if (nbytes > smallbuf.bytes) {
     MK_CHECK_HIP(hipStreamSynchronize(stream));
     MK_TRY(smallbuf.alloc(std::max<size_t>(nbytes, 1 << 20)));
@@ -63,12 +65,25 @@ mk_status Ctx::small(size_t nbytes, void** out) {
 }
 
 
-// Carve several regions out of the context arena.
+FixedScratch& fixed_scratch() {
+  static thread_local FixedScratch f;
+  return f;
+}
+
+// Carve several regions out of the context arena (or out of the calling
+// thread's fixed scratch region, if one is installed).
 mk_status carve(Ctx* ctx, std::initializer_list<std::pair<size_t, void**>> regions) {
   size_t total = 0;
   for (auto& r : regions) total += align256(r.first);
   void* base = nullptr;
-  MK_TRY(ctx->scratch(total, &base));
+  const FixedScratch& fx = fixed_scratch();
+  if (fx.base) {
+    MK_REQUIRE(total <= fx.bytes, MK_BAD_SHAPE,
+               "this apply needs %zu bytes of scratch; the fixed scratch holds %zu", total, fx.bytes);
+    base = fx.base;
+  } else {
+    MK_TRY(ctx->scratch(total, &base));
+  }
   size_t off = 0;
   for (auto& r : regions) {
     *r.second = static_cast<char*>(base) + off;
@@ -172,6 +187,8 @@ extern "C" mk_status mk_ctx_create(int device, mk_ctx* out) {
   MK_CHECK_HIP(hipGetDeviceCount(&n));
   MK_REQUIRE(device >= 0 && device < n, MK_BAD_ARG, "mk_ctx_create: device %d of %d", device, n);
   MK_CHECK_HIP(hipSetDevice(device));
+  // the allocation-free apply's scratch array: its address on this device
+  MK_TRY(noalloc_setup(device));
   auto* c = new Ctx();
   c->device = device;
   c->stream = nullptr;

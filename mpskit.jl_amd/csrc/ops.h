@@ -40,6 +40,27 @@ struct Ctx {
 };
 
 mk_status carve(Ctx* ctx, std::initializer_list<std::pair<size_t, void**>> regions);
+
+// A fixed scratch region for the calling host thread.  While one is installed
+// (FixedScratchGuard), carve() takes its regions from it instead of the
+// context arena and never allocates: a request larger than the region fails
+// with MK_BAD_SHAPE.  Ctx::small() refuses to run while one is installed.
+struct FixedScratch {
+  char* base = nullptr;
+  size_t bytes = 0;
+};
+FixedScratch& fixed_scratch();
+struct FixedScratchGuard {
+  explicit FixedScratchGuard(void* base, size_t bytes) {
+    fixed_scratch() = FixedScratch{static_cast<char*>(base), bytes};
+  }
+  ~FixedScratchGuard() { fixed_scratch() = FixedScratch{}; }
+  FixedScratchGuard(const FixedScratchGuard&) = delete;
+  FixedScratchGuard& operator=(const FixedScratchGuard&) = delete;
+};
+// Resolves the device address of the allocation-free entry point's scratch
+// array on `device` (which must be current) once per device; noalloc.hip.
+mk_status noalloc_setup(int device);
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 int64_t pad8(int64_t k);
 

@@ -2,6 +2,8 @@
 // pair on its own stream; durations are resolved when queried.
 #include "prof.h"
 
+#include "ops.h"
+
 #include <map>
 #include <mutex>
 #include <string>
@@ -39,7 +41,8 @@ ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
     std::lock_guard<std::mutex> lk(g_mu);
     on = g_on;
   }
-  if (!on) return;
+  // the allocation-free entry point creates no events
+  if (!on || fixed_scratch().base) return;
   // launches being captured into a graph are not timed
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
