@@ -69,6 +69,55 @@ __device__ __forceinline__ T* resolve(const GemmGroup& g, T* p) {
   return as_global(r);
 }
 
+// Epilogue C = alpha * acc + beta * C of an FM x FN grid of 16 x 16 accumulator
+// blocks, item r of lane (li, lq) at C[row0 + 16 i + li][col0 + 16 j + cq(lq, r)]
+// with cq = lq + 4 r (QMAJOR = false: the fp64 MFMA layout) or 4 lq + r
+// (QMAJOR = true: the fp32-accumulating MFMA layouts).  With beta != 0 the
+// 4 JB values of C of JB consecutive blocks of a row block are all fetched
+// before any of them is stored, so their reads overlap (one round trip per
+// batch instead of one per element; JB = 2 keeps the batch within the
+// registers the main loop leaves free); out-of-range items read a clamped
+// in-range address (branch-free) and are not stored.  beta == 0 never reads C.
+template <bool QMAJOR, int FM, int FN, class V, int JB = 2>
+__device__ __forceinline__ void epilogue_c(const V (&acc)[FM][FN], double* C, int64_t ldc, int M, int N, int row0,
+                                           int col0, int li, int lq, double alpha, double beta) {
+  static_assert(FN % JB == 0, "batch must divide the block columns");
+  gptr_t<double> Cg = gptr(C);
+  auto col_of = [&](int j, int r) { return col0 + j * 16 + (QMAJOR ? 4 * lq + r : lq + 4 * r); };
+  const bool rd = beta != 0.0;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int row = row0 + i * 16 + li;
+#pragma unroll
+    for (int j0 = 0; j0 < FN; j0 += JB) {
+      double cin[JB * 4];
+      if (rd) {
+        const int64_t rc = min(row, M - 1);
+#pragma unroll
+        for (int j = 0; j < JB; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) cin[j * 4 + r] = Cg[rc + (int64_t)min(col_of(j0 + j, r), N - 1) * ldc];
+      }
+      if (row >= M) continue;
+#pragma unroll
+      for (int j = 0; j < JB; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = col_of(j0 + j, r);
+          if (col >= N) continue;
+          double v = alpha * (double)acc[i][j0 + j][r];
+          if (rd) v += beta * cin[j * 4 + r];
+          Cg[(int64_t)row + (int64_t)col * ldc] = v;
+        }
+    }
+  }
+  // Drain the stores here.  Their data registers are reused inside the next
+  // tile's K loop, and without this wait the compiler's wait-count analysis
+  // carries the pending stores around the tile loop and inserts a full
+  // vmcnt(0) wait (on the K step's own global loads) into every K step.
+  __builtin_amdgcn_s_waitcnt(0);
+}
+
 // Position of tile u of a problem: tile rows are grouped GROUP_M at a time so
 // that consecutive tiles reuse the same A row panels.
 constexpr int GROUP_M = 8;

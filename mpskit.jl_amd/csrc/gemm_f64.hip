@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 namespace mk {
 namespace {
@@ -220,26 +221,247 @@ void gemm_f64_kernel(Src src) {
   // at block row lk + 4 r, column li; with the B fragment as first operand the
   // block is C^T, so the lane holds C[row li][col lk + 4 r]: the 16 lanes of a
   // quarter-wave store 16 consecutive rows (128 B) of one column.
-  const double alpha = P.alpha, beta = P.beta;
-  __attribute__((address_space(1))) double* Cp = (__attribute__((address_space(1))) double*)resolve(src, P.C);
+  epilogue_c<false, FM, FN, dx4, 1>(acc, resolve(src, P.C), P.ldc, P.M, P.N, m0 + wm0, n0 + wn0, li, lk, P.alpha, P.beta);
+  }  // tiles
+}
+
+// ---------------------------------------------------------------------------
+// Same tile, waves, LDS layout and epilogue as gemm_f64_kernel (BK = 16), with
+// a branch-free load path: each operand is read through a buffer resource based
+// at the tile's first row, the per-lane byte offsets of a segment are computed
+// once per segment, and a chunk outside the operand (row past M / N, k past K)
+// gets an offset past num_records, for which the hardware returns zeros.  A K
+// step then issues its eight 16-byte loads with two VALU ops each and no
+// exec-mask branches.  Needs (checked on the host, buf_ok): 16-byte aligned
+// operands, even leading dimensions, even K, an even row count for operands
+// stored along their rows, and every in-tile offset below 2 GiB.
+// ---------------------------------------------------------------------------
+constexpr unsigned OOB = 0x80000000u;
+typedef unsigned int ux4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+
+template <int BM, int BN, bool TA, bool TB, class Src, bool ROT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
+void gemm_f64_buf_kernel(Src src) {
+  constexpr int BK = 16;
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int CHA = BM * BK / 2 / NT;
+  constexpr int CHB = BN * BK / 2 / NT;
+  static_assert(CHA >= 1 && CHB >= 1, "tile too small for 256 threads");
+  // A is K-contiguous when transposed, B when not
+  constexpr bool AK = TA, BKD = !TB;
+
+  __shared__ double As[2][BK][LDA_S];
+  __shared__ double Bs[2][BK][LDB_S];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+  const int li = lane & 15;
+  const int lk = lane >> 4;
+
+  for (int t = src_first(src); t < src.total_tiles; t += src_stride(src)) {
+  const GemmProb& P = src_prob(src, t);
+  int tm, tn;
+  tile_coords(P, t - P.tile_start, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int M = P.M, N = P.N;
+
+  dx4 acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < FM; ++i) {
-    const int row = m0 + wm0 + i * 16 + li;
-    if (row >= P.M) continue;
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
+    for (int j = 0; j < FN; ++j) acc[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
+
+  int nsteps = 0;
+  for (int s = 0; s < P.nseg; ++s) nsteps += (P.seg[s].K + BK - 1) / BK;
+
+  // Chunk c of a thread (the mapping of load_fastdir / load_kdir):
+  //   along rows (R rows): rows r0, r0 + 1 with r0 = 2 (t % (R/2)), depth
+  //     t / (R/2) + c NT / (R/2): one row pair for all c, depth steps uniform;
+  //   along k: row t / 8 + 32 c, depths 2 (t % 8), + 1: one depth pair for
+  //     all c, row steps uniform.
+  // So a segment keeps one byte offset per operand (chunk 0 at k0 = 0, OOB
+  // if its rows lie outside the operand) and uniform strides; the per-chunk
+  // offsets and the bounds tests are formed at each load.  (Per-chunk offset
+  // arrays held 16 more registers through the K loop, which at this kernel's
+  // 256-register budget made the allocator alias load destinations and wait
+  // on the global loads inside the step.)
+  constexpr int RA2 = BM / 2, RB2 = BN / 2;
+  const int tid = threadIdx.x;
+  __amdgpu_buffer_rsrc_t rA, rB;
+  unsigned oA = 0, oB = 0;         // chunk 0, k0 = 0
+  unsigned cA = 0, cB = 0;         // bytes from chunk c to chunk c + 1
+  unsigned stepA = 0, stepB = 0;   // bytes per unit of k0
+  int K = 0;
+  auto set_seg = [&](int s) {
+    const GemmSeg& S = P.seg[s];
+    const char* pa = static_cast<const char*>(resolve(src, S.A));
+    const char* pb = static_cast<const char*>(resolve(src, S.B));
+    rA = raw_rsrc(pa + (AK ? (int64_t)m0 * S.lda : (int64_t)m0) * 8);
+    rB = raw_rsrc(pb + (BKD ? (int64_t)n0 * S.ldb : (int64_t)n0) * 8);
+    stepA = AK ? 8u : (unsigned)(S.lda * 8);
+    stepB = BKD ? 8u : (unsigned)(S.ldb * 8);
+    if (AK) {
+      cA = (unsigned)((NT / 8) * S.lda * 8);
+      oA = (unsigned)((2 * (tid % 8) + (int64_t)(tid / 8) * S.lda) * 8);
+    } else {
+      cA = (unsigned)((NT / RA2) * S.lda * 8);
+      oA = m0 + 2 * (tid % RA2) < M ? (unsigned)((2 * (tid % RA2) + (int64_t)(tid / RA2) * S.lda) * 8) : OOB;
+    }
+    if (BKD) {
+      cB = (unsigned)((NT / 8) * S.ldb * 8);
+      oB = (unsigned)((2 * (tid % 8) + (int64_t)(tid / 8) * S.ldb) * 8);
+    } else {
+      cB = (unsigned)((NT / RB2) * S.ldb * 8);
+      oB = n0 + 2 * (tid % RB2) < N ? (unsigned)((2 * (tid % RB2) + (int64_t)(tid / RB2) * S.ldb) * 8) : OOB;
+    }
+    K = S.K;
+  };
+
+  dx2 ra[CHA], rb[CHB];
+  auto load = [&](int k0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int col = n0 + wn0 + j * 16 + lk + 4 * r;
-        if (col >= P.N) continue;
-        __attribute__((address_space(1))) double* cp = Cp + (int64_t)row + (int64_t)col * P.ldc;
-        double v = alpha * acc[i][j][r];
-        if (beta != 0.0) v += beta * *cp;
-        *cp = v;
+    for (int c = 0; c < CHA; ++c) {
+      bool ok;
+      if (AK) ok = (m0 + tid / 8 + c * (NT / 8) < M) & (k0 + 2 * (tid % 8) < K);
+      else    ok = (oA != OOB) & (k0 + tid / RA2 + c * (NT / RA2) < K);
+      const unsigned o = ok ? oA + (unsigned)c * cA + (unsigned)k0 * stepA : OOB;
+      ra[c] = __builtin_bit_cast(dx2, __builtin_amdgcn_raw_buffer_load_b128(rA, (int)o, 0, 0));
+    }
+#pragma unroll
+    for (int c = 0; c < CHB; ++c) {
+      bool ok;
+      if (BKD) ok = (n0 + tid / 8 + c * (NT / 8) < N) & (k0 + 2 * (tid % 8) < K);
+      else     ok = (oB != OOB) & (k0 + tid / RB2 + c * (NT / RB2) < K);
+      const unsigned o = ok ? oB + (unsigned)c * cB + (unsigned)k0 * stepB : OOB;
+      rb[c] = __builtin_bit_cast(dx2, __builtin_amdgcn_raw_buffer_load_b128(rB, (int)o, 0, 0));
+    }
+  };
+  auto store = [&](int buf) {
+    if (!TA) store_fastdir<BM, LDA_S, BK>(As[buf], ra);
+    else     store_kdir<BM, LDA_S, BK>(As[buf], ra);
+    if (!TB) store_kdir<BN, LDB_S, BK>(Bs[buf], rb);
+    else     store_fastdir<BN, LDB_S, BK>(Bs[buf], rb);
+  };
+
+  auto read_frag = [&](int bi, int kk, double (&a)[FM], double (&b)[FN]) {
+    const int kr = kk * 4 + lk;
+    const int sa = TA ? kswz(kr) : 0;   // K-contiguous operands were stored swizzled
+    const int sb = TB ? 0 : kswz(kr);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) a[i] = As[bi][kr][(wm0 + i * 16 + li) ^ sa];
+#pragma unroll
+    for (int j = 0; j < FN; ++j) b[j] = Bs[bi][kr][(wn0 + j * 16 + li) ^ sb];
+  };
+  double fa[2][FM], fb[2][FN];
+
+  int s = 0, k0 = 0;
+  if (nsteps > 0) {
+    set_seg(0);
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  if (ROT && nsteps > 0) read_frag(0, 0, fa[0], fb[0]);
+  int buf = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    k0 += BK;
+    while (k0 >= K && s + 1 < P.nseg) {
+      ++s;
+      k0 = 0;
+      set_seg(s);
+    }
+    const bool has_next = step + 1 < nsteps;
+    if (has_next) load(k0);
+    if constexpr (!ROT) {
+#pragma unroll
+      for (int kk = 0; kk < BK / 4; ++kk) {
+        double a[FM], b[FN];
+        read_frag(buf, kk, a, b);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+      }
+      if (has_next) store(buf ^ 1);
+      __syncthreads();
+    } else {
+      // fragments of k slice kk + 1 are read while slice kk multiplies; the
+      // step barrier sits before the last slice's MFMAs, which then cover the
+      // latency of the next step's first fragment reads
+#pragma unroll
+      for (int kk = 0; kk < BK / 4; ++kk) {
+        if (kk + 1 < BK / 4) {
+          read_frag(buf, kk + 1, fa[(kk + 1) & 1], fb[(kk + 1) & 1]);
+        } else {
+          // (storing the next tile earlier in the step measured slower: its
+          // global loads, issued at the step start, are often still in flight)
+          if (has_next) store(buf ^ 1);
+          __builtin_amdgcn_sched_barrier(0);
+          __syncthreads();
+          __builtin_amdgcn_sched_barrier(0);
+          if (has_next) read_frag(buf ^ 1, 0, fa[0], fb[0]);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[kk & 1][j], fa[kk & 1][i], acc[i][j], 0, 0, 0);
       }
     }
+    buf ^= 1;
   }
+
+  // epilogue: as in gemm_f64_kernel
+  epilogue_c<false, FM, FN, dx4, 1>(acc, resolve(src, P.C), P.ldc, M, N, m0 + wm0, n0 + wn0, li, lk, P.alpha, P.beta);
   }  // tiles
+}
+
+// Host check of the branch-free kernel's conditions for one problem.
+template <int BM, int BN>
+bool buf_ok_prob(const ProbSpec& ps, GemmShape sh) {
+  for (int s = 0; s < ps.nseg; ++s) {
+    const SegSpec& g = ps.seg[s];
+    if (g.K <= 0) continue;
+    if ((uintptr_t)g.A % 16 || (uintptr_t)g.B % 16 || g.lda % 2 || g.ldb % 2 || g.K % 2) return false;
+    if (!sh.ta && ps.M % 2) return false;
+    if (sh.tb && ps.N % 2) return false;
+    // largest in-tile byte offset (plus the 16-byte chunk) below 2^31
+    const int64_t offA = sh.ta ? (g.K + (int64_t)(BM - 1) * g.lda) * 8 : (BM + (int64_t)(g.K - 1) * g.lda) * 8;
+    const int64_t offB = !sh.tb ? (g.K + (int64_t)(BN - 1) * g.ldb) * 8 : (BN + (int64_t)(g.K - 1) * g.ldb) * 8;
+    if (offA + 16 >= (1ll << 31) || offB + 16 >= (1ll << 31)) return false;
+  }
+  return true;
+}
+
+// Load path of the FP64 kernels, read once from MK_F64_VARIANT ("rot", "buf"
+// or "branch"; default "rot"): lets benchmarks and tests pin one of them.
+enum F64Variant { F64_BRANCH = 0, F64_BUF = 1, F64_ROT = 2 };
+int f64_variant() {
+  static const int v = [] {
+    const char* e = getenv("MK_F64_VARIANT");
+    if (e && !strcmp(e, "branch")) return (int)F64_BRANCH;
+    if (e && !strcmp(e, "buf")) return (int)F64_BUF;
+    return (int)F64_ROT;
+  }();
+  return v;
+}
+
+template <int BM, int BN, class Src, bool ROT>
+void launch4_buf(GemmShape sh, dim3 grid, dim3 block, hipStream_t stream, const Src& b) {
+  if (!sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_buf_kernel<BM, BN, false, false, Src, ROT>), grid, block, 0, stream, b);
+  else if (sh.ta && !sh.tb) hipLaunchKernelGGL((gemm_f64_buf_kernel<BM, BN, true, false, Src, ROT>), grid, block, 0, stream, b);
+  else if (!sh.ta && sh.tb) hipLaunchKernelGGL((gemm_f64_buf_kernel<BM, BN, false, true, Src, ROT>), grid, block, 0, stream, b);
+  else hipLaunchKernelGGL((gemm_f64_buf_kernel<BM, BN, true, true, Src, ROT>), grid, block, 0, stream, b);
 }
 
 // K depth of a stage, read once from MK_F64_BK (16 or 32; default 16).  BK = 16
@@ -297,9 +519,16 @@ mk_status launch_cfg(const ProbSpec* probs, int nprob, GemmShape sh, hipStream_t
   b.nprob = nprob;
   b.total_tiles = total;
   if (total == 0) return MK_OK;
+  const int var = f64_variant();
+  bool buf = var != F64_BRANCH && f64_bk() == 16;
+  for (int i = 0; i < nprob && buf; ++i) buf = buf_ok_prob<BM, BN>(probs[i], sh);
   dim3 grid(total), block(NT);
   ProfScope prof("gemm_f64", stream);
-  if (f64_bk() == 16) launch4<BM, BN, GemmBatch, 16>(sh, grid, block, stream, b);
+  // the rotated schedule holds two fragment sets; with a K-contiguous A the
+  // extra registers would spill, so those launches keep the plain schedule
+  if (buf && var == F64_ROT && !sh.ta) launch4_buf<BM, BN, GemmBatch, true>(sh, grid, block, stream, b);
+  else if (buf) launch4_buf<BM, BN, GemmBatch, false>(sh, grid, block, stream, b);
+  else if (f64_bk() == 16) launch4<BM, BN, GemmBatch, 16>(sh, grid, block, stream, b);
   else launch4<BM, BN, GemmBatch, BK32>(sh, grid, block, stream, b);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;

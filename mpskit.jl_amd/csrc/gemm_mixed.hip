@@ -55,23 +55,7 @@ int variant() {
 template <int FM, int FN>
 __device__ __forceinline__ void store_tile(const fx4 (&acc)[FM][FN], double* __restrict__ C, int64_t ldc, int M,
                                            int N, int row0, int col0, int li, int lg, double alpha, double beta) {
-#pragma unroll
-  for (int i = 0; i < FM; ++i) {
-    const int row = row0 + i * 16 + li;
-    if (row >= M) continue;
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int col = col0 + j * 16 + lg * 4 + r;
-        if (col >= N) continue;
-        gptr_t<double> cp = gptr(C) + (int64_t)row + (int64_t)col * ldc;
-        double v = alpha * (double)acc[i][j][r];
-        if (beta != 0.0) v += beta * *cp;
-        *cp = v;
-      }
-    }
-  }
+  epilogue_c<true>(acc, C, ldc, M, N, row0, col0, li, lg, alpha, beta);
 }
 
 // Load a (R rows x 64 B) slab of a K-contiguous operand into registers.
