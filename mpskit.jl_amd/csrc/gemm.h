@@ -58,6 +58,7 @@ struct GemmGroup {
   int32_t total_tiles;
   const char* base[NBASE];
   int32_t rows256_fit;   // 256 rows of every mixed operand span < 2 GiB (32-bit buffer offsets)
+  int32_t f64_buf;       // every FP64 problem meets the branch-free kernel's conditions (f64_buf_ok)
 };
 
 constexpr uint64_t REL_MASK = (1ull << 62) - 1;
@@ -118,6 +119,11 @@ bool tile_valid(int prec, int kind);
 // Workgroups of one persistent launch: resident workgroups per CU x CUs, capped by the tiles.
 int group_grid(int prec, int kind, GemmShape sh, int total_tiles);
 mk_status gemm_f64_group(const GemmGroup& g, int kind, GemmShape sh, int grid, hipStream_t st);
+// Whether the FP64 problems may run on the branch-free (buffer-load) kernel
+// with tile kind `kind`: 16-byte aligned operands (relative pointers: their
+// offsets), even leading dimensions and K, even row counts for operands stored
+// along their rows, in-tile offsets below 2 GiB.
+bool f64_buf_ok(const ProbSpec* probs, int n, GemmShape sh, int kind);
 mk_status gemm_mixed_group(const GemmGroup& g, int prec, int kind, int grid, hipStream_t st);
 
 }  // namespace mk

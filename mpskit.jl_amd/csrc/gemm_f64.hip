@@ -539,7 +539,11 @@ mk_status launch_group(const GemmGroup& g, GemmShape sh, int grid, hipStream_t s
   if (g.total_tiles == 0) return MK_OK;
   dim3 gr(grid), block(NT);
   ProfScope prof("gemm_f64", stream);
-  if (f64_bk() == 16) launch4<BM, BN, GemmGroup, 16>(sh, gr, block, stream, g);
+  const int var = f64_variant();
+  const bool buf = g.f64_buf && var != F64_BRANCH && f64_bk() == 16;
+  if (buf && var == F64_ROT && !sh.ta) launch4_buf<BM, BN, GemmGroup, true>(sh, gr, block, stream, g);
+  else if (buf) launch4_buf<BM, BN, GemmGroup, false>(sh, gr, block, stream, g);
+  else if (f64_bk() == 16) launch4<BM, BN, GemmGroup, 16>(sh, gr, block, stream, g);
   else launch4<BM, BN, GemmGroup, BK32>(sh, gr, block, stream, g);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
@@ -565,6 +569,12 @@ int occupancy() {
 int f64_group_grid(int kind, int total_tiles, int cus) {
   const int occ = kind == TK_128 ? occupancy<128, 128>() : occupancy<64, 64>();
   return (int)std::min<int64_t>(total_tiles, (int64_t)occ * cus);
+}
+
+bool f64_buf_ok(const ProbSpec* probs, int n, GemmShape sh, int kind) {
+  for (int i = 0; i < n; ++i)
+    if (!(kind == TK_128 ? buf_ok_prob<128, 128>(probs[i], sh) : buf_ok_prob<64, 64>(probs[i], sh))) return false;
+  return true;
 }
 
 mk_status gemm_f64_group(const GemmGroup& g, int kind, GemmShape sh, int grid, hipStream_t st) {

@@ -146,6 +146,9 @@ mk_status GroupPlan::build(int k, hipStream_t launched_on) {
   kind = k;
   nprob = (int)host.size();
   total_tiles = total;
+  // relative pointers keep their offsets in the low bits and the base slots
+  // are 16-byte aligned, so the alignment test reads the offsets
+  f64_buf = prec == MK_FP64 && (k == TK_64 || k == TK_128) && f64_buf_ok(specs.data(), (int)specs.size(), sh, k);
   grid = group_grid(prec, k, sh, total);
   return MK_OK;
 }
@@ -193,6 +196,7 @@ mk_status GroupPlan::run(const void* const base[NBASE], hipStream_t st) const {
   g.nprob = nprob;
   g.total_tiles = total_tiles;
   const int64_t eb = prec == MK_FP32 ? 4 : 2;
+  g.f64_buf = f64_buf ? 1 : 0;
   g.rows256_fit = 1;
   for (const ProbSpec& p : specs)
     for (int s = 0; s < p.nseg; ++s)

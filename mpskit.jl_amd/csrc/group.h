@@ -29,6 +29,7 @@ struct GroupPlan {
   int total_tiles = 0;
   int grid = 0;
   double flops = 0.0;
+  bool f64_buf = false;         // FP64 problems fit the branch-free kernel (set by build)
   std::vector<ProbSpec> specs;  // problems in launch order (largest per-tile work first)
   DevBuf dev;                   // GemmProb array
   // fill and upload the descriptors for one tile kind; a rebuild first waits for
