@@ -33,8 +33,17 @@ namespace mk {
 void lanczos_lowest(const std::vector<double>& G, int ld, int k, double& lam, std::vector<double>& s);
 void lanczos_lowest_p(const std::vector<double>& G, int ld, int k, int p, std::vector<double>& lam,
                       std::vector<double>& S);
+bool lanczos_local_first_pass();
 
 namespace {
+
+// First basis column of the first Gram-Schmidt pass of krylov_step(j): the
+// whole basis for the first Krylov vector of a cycle (it couples to x and the
+// kept Ritz vectors), else the last three columns (the three-term recurrence;
+// the full second pass removes the rest).
+int first_pass_lo(int j, bool first) {
+  return (first || !lanczos_local_first_pass()) ? 0 : std::max(0, j - 1);
+}
 
 struct PrecPlan {
   bool ready = false;
@@ -92,7 +101,7 @@ struct Batch {
     return MK_OK;
   }
   mk_status apply(const double* x, double* y, int prec, hipStream_t st);
-  mk_status krylov_step(int j, int prec, hipStream_t st);
+  mk_status krylov_step(int j, int prec, bool first, hipStream_t st);
 };
 
 namespace {
@@ -247,8 +256,9 @@ mk_status Batch::apply(const double* x, double* y, int prec, hipStream_t st) {
 // Device work of Krylov step j: U = H_p v_j, two Gram-Schmidt passes against
 // Q[:, 0..j+1], |U|^2, v_{j+1} = U / |U| and <v_{j+1}, W>; coefficients are
 // copied to the pinned host buffer.  Layout of `coef`: c1 | c2 (nseg x ks each),
-// nn | xw (nseg each).
-mk_status Batch::krylov_step(int j, int prec, hipStream_t st) {
+// nn | xw (nseg each).  The first pass covers columns first_pass_lo(j)..j+1
+// (entries of c1 below it are stale and read as zero).
+mk_status Batch::krylov_step(int j, int prec, bool first, hipStream_t st) {
   double* Qp = static_cast<double*>(Q.ptr);
   double* Up = static_cast<double*>(U.ptr);
   double* Wp = static_cast<double*>(W.ptr);
@@ -261,8 +271,9 @@ mk_status Batch::krylov_step(int j, int prec, hipStream_t st) {
   double* xw = nn + ns;
   auto col = [&](int q) { return Qp + (int64_t)q * total; };
   MK_TRY(apply(col(j + 1), Up, prec, st));
-  MK_TRY(seg_dots(sg, Qp, total, j + 2, Up, c1, ks, part, st));
-  MK_TRY(seg_multiaxpy(sg, Qp, total, j + 2, c1, ks, Up, st));
+  const int lo = first_pass_lo(j, first);
+  MK_TRY(seg_dots(sg, col(lo), total, j + 2 - lo, Up, c1 + lo, ks, part, st));
+  MK_TRY(seg_multiaxpy(sg, col(lo), total, j + 2 - lo, c1 + lo, ks, Up, st));
   MK_TRY(seg_dots(sg, Qp, total, j + 2, Up, c2, ks, part, st));
   MK_TRY(seg_multiaxpy(sg, Qp, total, j + 2, c2, ks, Up, st));
   MK_TRY(seg_dots(sg, Up, total, 1, Up, nn, 1, part, st));
@@ -569,13 +580,13 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
     for (int j = 0; j < mmax; ++j) {
       const int jj = p + j - 1;   // krylov_step(jj) applies column jj + 1
       if (graphs) {
-        const auto key = std::make_pair(precision, jj);
+        const auto key = std::make_pair(precision, 2 * jj + (j == 0 ? 1 : 0));
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
           hipGraph_t g = nullptr;
           hipGraphExec_t ex = nullptr;
           MK_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-          const mk_status s = b->krylov_step(jj, precision, st);
+          const mk_status s = b->krylov_step(jj, precision, j == 0, st);
           const hipError_t e = hipStreamEndCapture(st, &g);
           MK_TRY(s);
           MK_CHECK_HIP(e);
@@ -586,7 +597,7 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
         }
         MK_CHECK_HIP(hipGraphLaunch(it->second, st));
       } else {
-        MK_TRY(b->krylov_step(jj, precision, st));
+        MK_TRY(b->krylov_step(jj, precision, j == 0, st));
       }
       MK_CHECK_HIP(hipStreamSynchronize(st));
       const double* c1 = hb;
@@ -595,11 +606,14 @@ mk_status mk_batch_eigsolve(mk_batch h, double* x, int krylovdim, int maxiter, d
       const double* xw = nn + n;
       const int cj = p + j;
       bool still = false;
+      const int lo1 = first_pass_lo(jj, j == 0);
       for (int i = 0; i < n; ++i) {
         if (!active[i]) continue;
         ++ops_n[i];
         std::vector<double>& Gi = G[i];
-        for (int r = 1; r <= cj; ++r) Gi[(size_t)r * ldg + cj] = Gi[(size_t)cj * ldg + r] = c1[(int64_t)i * kst + r] + c2[(int64_t)i * kst + r];
+        for (int r = 1; r <= cj; ++r)
+          Gi[(size_t)r * ldg + cj] = Gi[(size_t)cj * ldg + r] =
+              (r >= lo1 ? c1[(int64_t)i * kst + r] : 0.0) + c2[(int64_t)i * kst + r];
         const double beta = std::sqrt(std::max(nn[i], 0.0));
         MK_REQUIRE(std::isfinite(beta), MK_NONFINITE, "batch eigsolve: non-finite Krylov vector %d", i);
         double lam;

@@ -11,7 +11,9 @@
 // lowest Ritz vectors kept from the previous cycle: a thick restart, as
 // KrylovKit's Lanczos keeps part of its subspace) by a Krylov space of the
 // (possibly mixed-precision) operator started from r, orthonormalised in
-// float64 (two classical Gram-Schmidt passes), and takes the lowest Ritz pair
+// float64 (two classical Gram-Schmidt passes, of which the first covers only
+// the last three columns after the first Krylov vector of a cycle: the
+// three-term recurrence), and takes the lowest Ritz pair
 // of the projected matrix, whose x-row <q, H x> uses the float64 w.  Keeping
 // Ritz vectors carries the convergence of the cycles before, which a restart
 // from x alone loses: with eigenvalue gaps small against the spectral width
@@ -24,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "gemm.h"
@@ -111,6 +114,19 @@ static double inner_eta(int precision) {
   }();
   return precision == MK_FP64 ? 0.0 : eta;
 }
+
+// Krylov steps after the first of a cycle take their first orthogonalisation
+// pass on the last three basis columns (the three-term recurrence) instead of
+// the whole basis; MK_LANCZOS_CGS=full restores two full passes (read once).
+static bool local_first_pass() {
+  static const bool on = [] {
+    const char* e = getenv("MK_LANCZOS_CGS");
+    return !(e && std::string(e) == "full");
+  }();
+  return on;
+}
+
+bool lanczos_local_first_pass() { return local_first_pass(); }
 
 }  // namespace mk
 
@@ -266,7 +282,19 @@ extern "C" mk_status mk_eigsolve(mk_op h, double* x, int krylovdim, int maxiter,
         // fused: c1 = Q^T u; u -= Q c1 with c2 = Q^T u in the same read of Q;
         // u -= Q c2 with |u|^2 and <w, u>; one readback for the step
         double* cd = dv.coef + FC;
-        MK_TRY(k_multidot(Q, n, kq, u, n, cd, dv.partial, st));
+        if (j >= 1 && local_first_pass()) {
+          // first pass on the last three columns only.  For j >= 1, H v_j lies
+          // in span{v_{j-1}, v_j, v_{j+1}} in exact arithmetic (x and the kept
+          // Ritz vectors couple to v_0 alone), so the three-term recurrence
+          // removes the large components; the full pass below removes what is
+          // left, which is small, so the pair is as orthogonal as two full
+          // passes (twice is enough) for one read of the columns instead of two
+          MK_CHECK_HIP(hipMemsetAsync(cd, 0, (size_t)kq * 8, st));
+          const int lo = std::max(0, cj - 2);
+          MK_TRY(k_multidot(col(lo), n, cj + 1 - lo, u, n, cd + lo, dv.partial, st));
+        } else {
+          MK_TRY(k_multidot(Q, n, kq, u, n, cd, dv.partial, st));
+        }
         MK_TRY(k_cgs_update_dot(Q, n, kq, cd, u, n, cd + 32, dv.partial, st));
         MK_TRY(k_cgs_update_norm(Q, n, kq, cd + 32, u, w, n, cd + 64, dv.partial, st));
         std::vector<double> hv(2 * 32 + 2);
