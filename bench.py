@@ -14,8 +14,9 @@ each against the FP64 result, the GEMM roofline measured with HIP events, and
 the float64 numpy/OpenBLAS restatement of the same apply timed on host cores.
 
 The same line carries the DMRG2 sweep time at D=2048: one sweep of the
-Heisenberg chain (L=24, bonds up to 2048, random normalised start, seed 0) in
-FP64 and in the mixed path, with the energies of both.
+Heisenberg chain (L=64, 43 of its 63 bonds at 2048, truncating from 4096;
+random normalised start, seed 0) in FP64 and in the mixed path, with the
+energies of both (--sweep-L sets the chain length, --no-sweep skips it).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--D 2048] [--no-cpu] [--no-sweep]
 
@@ -112,6 +113,50 @@ def vendor_gemm_reference(D, d=2, reps=20):
     return out
 
 
+def apply_table(mk, T, steps=5):
+    """applies/s of the other operators and sizes of the isolated-apply workload
+    beside the headline: the Jordan dAC2
+    at D = 512, 1024, 4096, and at D = 2048 the generic MPO dAC2 (w = 5, the
+    same W taken as a dense MPO tensor) and the Jordan one-site dAC, in FP64,
+    FP32 and BF16x3 (inputs as in the headline workload, other seeds)."""
+    import torch
+    W = T.from_numpy(jordan_w(np.random.default_rng(3)))
+    cases = [("jordan_dAC2", D) for D in (512, 1024, 4096)] + [("generic_dAC2", 2048), ("jordan_dAC", 2048)]
+    out = {}
+    for kind, D in cases:
+        GL, GR = gpu_unit(T, (D, 5, D), 11), gpu_unit(T, (D, 5, D), 12)
+        if kind == "jordan_dAC2":
+            H = mk.JordanMPO_AC2_Hamiltonian(GL, W, W, GR)
+            shape = (D, 2, D, 2)
+        elif kind == "generic_dAC2":
+            H = mk.MPO_AC2_Hamiltonian(GL, W, W, GR)
+            shape = (D, 2, D, 2)
+        else:
+            H = mk.JordanMPO_AC_Hamiltonian(GL, W, GR)
+            shape = (D, 2, D)
+        op = mk.prepare_operator(H, precisions=("fp64", "fp32", "bf16x3"))
+        del GL, GR, H
+        x = gpu_unit(T, shape, 13)
+        y = T.empty(shape)
+        row = {}
+        y64 = None
+        for prec in ("fp64", "fp32", "bf16x3"):
+            ms = time_applies(op, x, y, prec, steps if D < 4096 else 3, 1)
+            if prec == "fp64":
+                y64 = y.clone()
+            row[prec] = {"applies_per_s": 1e3 / ms, "tflops": op.flops / (ms * 1e-3) / 1e12}
+            if y64 is not None and prec != "fp64":
+                row[prec]["rel_err_vs_fp64"] = float(torch.linalg.vector_norm(y - y64) / torch.linalg.vector_norm(y64))
+        row["bf16x3_speedup_vs_fp64"] = row["bf16x3"]["applies_per_s"] / row["fp64"]["applies_per_s"]
+        out[f"{kind} D={D}"] = row
+        print(f"# {kind} D={D}: " + ", ".join(f"{p} {row[p]['applies_per_s']:.1f}/s" for p in ("fp64", "fp32", "bf16x3")),
+              file=sys.stderr, flush=True)
+        op.free()
+        del x, y, y64
+        torch.cuda.empty_cache()
+    return out
+
+
 def batched_apply(mk, T, reps=10):
     """The grouped (batched) dAC2 path: one mk_batch_apply applies every
     instance of a batch in one grouped persistent GEMM launch plus one memory-
@@ -166,7 +211,10 @@ def cpu_baseline(D, budget_s=20.0):
     """float64 oracle restatement (numpy/OpenBLAS) of the same apply on host cores."""
     from threadpoolctl import threadpool_info, threadpool_limits
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
+    # the host share of one GPU is 16 threads (OMP_NUM_THREADS on the box);
+    # all of them run the BLAS products
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16") or 16), avail))
     rng = np.random.default_rng(5)
     d = 2
     W = jordan_w(rng)
@@ -188,7 +236,22 @@ def cpu_baseline(D, budget_s=20.0):
                       default=cores)
     return dict(value=n / el, unit="applies/s", cores=int(threads), kind="port",
                 sample=f"{n} applies of the prepared nearest-neighbour Jordan dAC2 at D={D} "
-                       f"(numpy/OpenBLAS float64: x*IC + BE*x + CB term), {el:.1f} s")
+                       f"(numpy/OpenBLAS float64: x*IC + BE*x + CB term), {el:.1f} s",
+                host=host_info(avail))
+
+
+def host_info(avail):
+    """What the host offers: logical CPUs, the ones this process may use, the model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": avail, "model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def dmrg_sweep(mk, prec, L=24, D=2048):
@@ -322,7 +385,8 @@ def u1_workload(args):
     """U(1) XXZ DMRG2 (L = 64, Sz = 0, D = 2048 over 21 charge sectors with a
     discretised Gaussian profile; interior sectors carry zero padding states up
     to a multiple of 8): sweep time and applies in float64 and in the mixed
-    path, energies of both."""
+    path, energies of both.  The local problems run on compact block storage
+    (--u1-storage dense: the dense charge-sorted layout)."""
     import torch
     import mpskit_amd as mk
     L, nup = args.L if args.L != 48 else 64, None
@@ -340,7 +404,7 @@ def u1_workload(args):
         H = mk.FiniteMPOHamiltonian(mk.models.xxz(L, 1.0))
         print(f"# u1 {prec} bonds {psi.bond_dims()}", file=sys.stderr, flush=True)
         hist = mk.find_groundstate_u1(psi, H, trunc_dim=2048, sweeps=args.u1_sweeps, precision=prec, tol=1e-10,
-                                      verbosity=1)
+                                      verbosity=1, storage=args.u1_storage)
         res[prec] = dict(sweeps=[dict(time=h["time"], solve_time=h["solve_time"], applies=h["numops"],
                                       energy_per_site=h["energy"] / L) for h in hist],
                          mid_bond=hist[-1]["mid_bond"])
@@ -350,7 +414,8 @@ def u1_workload(args):
         "value": last["bf16x3"]["time"], "unit": "s/sweep", "n_gpus": 1, "steps": args.u1_sweeps, "warmup": 0,
         "ms_per_step": last["bf16x3"]["time"] * 1e3, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16x3", "data": "synthetic",
-        "config": {"workload": f"U(1) XXZ DMRG2, L={L}, Sz=0, D=2048 over 21 sectors {prof}"},
+        "config": {"workload": f"U(1) XXZ DMRG2, L={L}, Sz=0, D=2048 over 21 sectors {prof}",
+                   "storage": args.u1_storage},
         "fp64": res["fp64"], "bf16x3": res["bf16x3"],
         "energy_diff_per_site": abs(last["fp64"]["energy_per_site"] - last["bf16x3"]["energy_per_site"]),
     }
@@ -359,7 +424,8 @@ def u1_workload(args):
 
 def sweep_workload(args):
     """Where one DMRG2 sweep at D=2048 spends its time: the sweep of the default
-    line, run once per precision with host timers (device synchronised) around
+    line at chain length --sweep-L (64: 43 of the 63 bonds at 2048, truncating
+    from 4096), run once per precision with host timers (device synchronised) around
     the local eigensolves, the truncated splits, the QR of the local error
     estimate, the environment updates and the operator preparation."""
     import collections
@@ -389,13 +455,13 @@ def sweep_workload(args):
     timed(env_mod, "transfer_left", "environment updates")
     timed(env_mod, "transfer_right", "environment updates")
     timed(alg_mod, "prepare_operator", "operator preparation")
-    out = {"metric": "DMRG2 sweep time breakdown, Heisenberg chain L=24, bonds up to 2048", "unit": "s",
+    out = {"metric": f"DMRG2 sweep time breakdown, Heisenberg chain L={args.sweep_L}, bonds up to 2048", "unit": "s",
            "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
            "config": {"workload": "one DMRG2 sweep from a random normalised state (seed 0), Lanczos tol 1e-10"}}
     for prec in ("fp64", "bf16x3"):
         acc.clear()
         cnt.clear()
-        sec, e, nops = dmrg_sweep(mk, prec)
+        sec, e, nops = dmrg_sweep(mk, prec, L=args.sweep_L)
         out[prec] = {"seconds": sec, "energy_per_site": e, "applies": nops,
                      "parts": {k: {"seconds": acc[k], "calls": cnt[k]} for k in sorted(acc, key=lambda k: -acc[k])}}
         print(f"# {prec}: sweep {sec:.2f} s", file=sys.stderr, flush=True)
@@ -487,8 +553,11 @@ def vumps_workload(args):
         alg = V.VUMPS(tol=1e-12, maxiter=iters, precision=prec, verbosity=2)
         _, _, eps, hist = V.find_groundstate_vumps(psi, H, alg)
         per = [h["time"] for h in hist[1:]] or [hist[0]["time"]]
+        last = [h["time"] for h in hist[-5:]]
         res[prec] = dict(seconds_per_iteration=sum(per) / len(per), energy_per_site=hist[-1]["energy"],
-                         galerkin=hist[-1]["eps"], iterations=len(hist))
+                         galerkin=hist[-1]["eps"], iterations=len(hist), total_seconds=sum(h["time"] for h in hist),
+                         seconds_per_iteration_last5=sum(last) / len(last),
+                         energies=[h["energy"] for h in hist], errors=[h["eps"] for h in hist])
     out = {
         "metric": "VUMPS time per iteration, infinite J1-J2 chain (J2/J1=0.24, cell 2, w=8)",
         "value": res["bf16x3"]["seconds_per_iteration"], "unit": "s/iteration", "n_gpus": 1, "steps": iters,
@@ -585,10 +654,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32", "bf16x3"], default=None,
                     help="time one precision only (for profiler runs); prints no JSON line")
-    ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the DMRG2 sweep at D=2048 (about 3 minutes)")
+    ap.add_argument("--no-table", action="store_true", help="skip the table of the other isolated applies")
     ap.add_argument("--workload", choices=["apply", "batched", "vumps", "u1", "sweep", "idmrg", "idmrg2", "boundary", "tdvp"], default="apply")
     ap.add_argument("--vumps-D", type=int, default=1024)
+    ap.add_argument("--sweep-L", type=int, default=64, help="chain length of the DMRG2 sweeps (default line and sweep workload)")
     ap.add_argument("--u1-sweeps", type=int, default=2)
+    ap.add_argument("--u1-storage", choices=["compact", "dense"], default="compact",
+                    help="charge-block storage of the U(1) workload's local problems")
     ap.add_argument("--vumps-iters", type=int, default=6)
     ap.add_argument("--idmrg2-D", type=int, default=256, help="kept states of the IDMRG2 workload")
     ap.add_argument("--boundary-D", type=int, default=1024, help="bond dimension of the boundary workload")
@@ -704,21 +777,25 @@ def main():
         "roofline_fp64": {"bound": "mfma", "achieved": F / (res["fp64"]["gemm_ms"] * 1e-3) / 1e12,
                           "peak": PEAK_TF["fp64"], "unit": "TFLOP/s",
                           "frac": F / (res["fp64"]["gemm_ms"] * 1e-3) / 1e12 / PEAK_TF["fp64"],
-                          "traffic": traffic_from_profile("gemm_f64_kernel<128,128>", D),
-                          "kernel": "gemm_f64_kernel<128,128>"},
+                          "traffic": traffic_from_profile("gemm_f64_buf_kernel<128,128>", D),
+                          "kernel": "gemm_f64_buf_kernel<128,128> (branch-free loads, rotated K step)"},
     }
     print("# vendor GEMM reference ...", file=sys.stderr, flush=True)
     out["vendor_gemm_reference"] = vendor_gemm_reference(D)
     print("# batched applies ...", file=sys.stderr, flush=True)
     out["batched_apply"] = batched_apply(mk, T)
+    if not args.no_table:
+        print("# apply table ...", file=sys.stderr, flush=True)
+        out["apply_table"] = apply_table(mk, T)
     if not args.no_sweep:
         sw = {}
         for prec in ("fp64", "bf16x3"):
-            print(f"# DMRG2 sweep {prec} ...", file=sys.stderr, flush=True)
-            sec, e, nops = dmrg_sweep(mk, prec)
+            print(f"# DMRG2 sweep {prec} (L={args.sweep_L}) ...", file=sys.stderr, flush=True)
+            sec, e, nops = dmrg_sweep(mk, prec, L=args.sweep_L)
             sw[prec] = dict(seconds=sec, energy_per_site=e, applies=nops)
-        out["dmrg_sweep"] = {"workload": "DMRG2, Heisenberg chain L=24, bonds up to 2048, one sweep from a random "
-                                         "normalised state (seed 0), Lanczos tol 1e-10",
+            print(f"# DMRG2 sweep {prec}: {sec:.2f} s", file=sys.stderr, flush=True)
+        out["dmrg_sweep"] = {"workload": f"DMRG2, Heisenberg chain L={args.sweep_L}, bonds up to 2048 (truncating from "
+                                         f"4096), one sweep from a random normalised state (seed 0), Lanczos tol 1e-10",
                              "fp64": sw["fp64"], "bf16x3": sw["bf16x3"],
                              "speedup": sw["fp64"]["seconds"] / sw["bf16x3"]["seconds"],
                              "energy_diff_per_site": abs(sw["fp64"]["energy_per_site"] -
