@@ -127,6 +127,24 @@ mk_status OpMat::make_mixed(int precisions, hipStream_t st) {
   return MK_OK;
 }
 
+mk_status OpMat::make_packed(int precisions, hipStream_t st) {
+  const bool kfast = (role == 'B');
+  const int64_t K = kfast ? rows : cols;
+  const int64_t MN = kfast ? cols : rows;
+  if ((precisions & MK_PREC_BF16X3) && hi.ptr) {
+    MK_TRY(phi.alloc((size_t)MN * K * 2));
+    MK_TRY(plo.alloc((size_t)MN * K * 2));
+    MK_TRY(k_repack(hi.ptr, MN, K, kpad, 2, phi.ptr, st));
+    MK_TRY(k_repack(lo.ptr, MN, K, kpad, 2, plo.ptr, st));
+  }
+  if ((precisions & MK_PREC_FP32) && f32.ptr) {
+    MK_TRY(pf32.alloc((size_t)MN * K * 4));
+    MK_TRY(k_repack(f32.ptr, MN, K, kpad, 4, pf32.ptr, st));
+  }
+  packed = true;
+  return MK_OK;
+}
+
 // ---------------------------------------------------------------------------
 // GEMM call helpers over OpMat / raw x views
 // ---------------------------------------------------------------------------

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "gemm.h"
+
 namespace mk {
 
 // bf16 hi/lo split of a float64 value: hi = bf16(v), lo = bf16(v - hi).
@@ -31,7 +33,10 @@ __device__ __forceinline__ void split2(double v, __hip_bfloat16& hi, __hip_bfloa
 constexpr int PRO_A = 32, PRO_C = 64;
 typedef unsigned int ProTile[4][PRO_C][PRO_A + 1];
 
-template <bool SPLIT>
+// PACK: the two views are written in the packed panel layout of the 256 x 256
+// mixed kernel (packed_offset, with ka = N and kb = M their K lengths) instead
+// of K-contiguous rows of length ka / kb.
+template <bool SPLIT, bool PACK = false>
 __device__ __forceinline__ void nn_prologue_tile(const double* __restrict__ P, const double* __restrict__ x,
                                                  int64_t Dl, int64_t Dr, double alpha, double beta,
                                                  double* __restrict__ y, void* __restrict__ xa_hi,
@@ -73,17 +78,18 @@ __device__ __forceinline__ void nn_prologue_tile(const double* __restrict__ P, c
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int64_t rr = a + Dl * (q & 1), cl = c + Dr * (q >> 1);
+          const int64_t ob = PACK ? packed_offset(cl, rr, kb, SPLIT ? 32 : 16) : cl * kb + rr;
           unsigned int packed;
           if constexpr (SPLIT) {
             __hip_bfloat16 h, l;
             split2(xv[q], h, l);
-            ((gus*)xb_hi)[cl * kb + rr] = __builtin_bit_cast(unsigned short, h);
-            ((gus*)xb_lo)[cl * kb + rr] = __builtin_bit_cast(unsigned short, l);
+            ((gus*)xb_hi)[ob] = __builtin_bit_cast(unsigned short, h);
+            ((gus*)xb_lo)[ob] = __builtin_bit_cast(unsigned short, l);
             packed = (unsigned int)__builtin_bit_cast(unsigned short, h) |
                      ((unsigned int)__builtin_bit_cast(unsigned short, l) << 16);
           } else {
             const float f = (float)xv[q];
-            ((gf*)xb_hi)[cl * kb + rr] = f;
+            ((gf*)xb_hi)[ob] = f;
             packed = __float_as_uint(f);
           }
           t[q][cc][tx] = packed;
@@ -103,7 +109,8 @@ __device__ __forceinline__ void nn_prologue_tile(const double* __restrict__ P, c
         const int64_t ar = a0 + aa;
         if (ar >= Dl || c >= Dr) continue;
         const unsigned int v = t[q][tx][aa];
-        const int64_t idx = (ar + Dl * (q & 1)) * ka + c + Dr * (q >> 1);
+        const int64_t row = ar + Dl * (q & 1), kk = c + Dr * (q >> 1);
+        const int64_t idx = PACK ? packed_offset(row, kk, ka, SPLIT ? 32 : 16) : row * ka + kk;
         if constexpr (SPLIT) {
           ((gus*)xa_hi)[idx] = (unsigned short)(v & 0xffffu);
           ((gus*)xa_lo)[idx] = (unsigned short)(v >> 16);

@@ -76,7 +76,21 @@ struct SegSpec {
   int64_t K;
   const void* Alo = nullptr;
   const void* Blo = nullptr;
+  // mixed precision only: the operand is stored in the packed panel layout
+  // (packed_offset below; lda / ldb then equal K)
+  bool packA = false, packB = false;
 };
+
+// Packed operand layout of the 256 x 256 mixed-precision kernel.  An operand
+// with MN rows (its non-K index, a multiple of 256) and K (a multiple of 32)
+// K-contiguous elements per row is stored as MN / 256 panels of K / KS slabs
+// (KS = 32 bf16 or 16 fp32 elements: 64 bytes); slab s of panel p holds the
+// 64 bytes of K step s of rows 256 p .. 256 p + 255, row after row.  A K step
+// of a tile is then one contiguous 16 KB read, and every 128-byte line it
+// touches is used in that step.  Element offset of (row r, k):
+__host__ __device__ inline int64_t packed_offset(int64_t r, int64_t k, int64_t K, int ks) {
+  return ((r >> 8) * (K / ks) + k / ks) * (256 * (int64_t)ks) + (r & 255) * ks + k % ks;
+}
 
 struct ProbSpec {
   int64_t M = 0, N = 0;
@@ -108,6 +122,9 @@ mk_status gemm_f64(const ProbSpec* probs, int nprob, GemmShape shape, hipStream_
 // accumulation in fp32, output C is float64 column-major (alpha/beta in fp64).
 mk_status gemm_bf16x3(const ProbSpec* probs, int nprob, hipStream_t stream);
 mk_status gemm_f32(const ProbSpec* probs, int nprob, hipStream_t stream);
+// Whether one M x N problem whose segments have the given K values runs on the
+// 256 x 256 kernel with packed operands (the only kernel that reads them).
+bool mixed_packed_ok(int64_t M, int64_t N, const int64_t* Ks, int nk);
 
 // Grouped (persistent) launches over a device descriptor array.  Tile kinds:
 //   FP64:  TK_64 (64x64), TK_128 (128x128)

@@ -414,24 +414,35 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big2_kernel(Src src) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
 
-  // per-segment load state
+  // per-segment load state.  A row-layout operand is addressed by row; a
+  // packed one (256-row panels of contiguous 16 KB K steps, every row present)
+  // by its panel, with the K step at 16 KB per 64 bytes of K.
   __amdgpu_buffer_rsrc_t rA[NPART], rB[NPART];
   unsigned vA[CH], vB[CH];
+  unsigned kmulA = 1, kmulB = 1;   // bytes of offset per byte of K
   int Kb = 0, kend = 0;
   auto set_seg = [&](int s) {
     const GemmSeg& S = P.seg[s];
+    const bool pa = (S.vec & 2) != 0, pb = (S.vec & 4) != 0;
+    const int64_t panel = (int64_t)S.K * EB * 256;
     const int64_t ldab = S.lda * EB, ldbb = S.ldb * EB;
-    rA[0] = raw_rsrc(static_cast<const char*>(resolve(src, S.A)) + (int64_t)m0 * ldab);
-    rB[0] = raw_rsrc(static_cast<const char*>(resolve(src, S.B)) + (int64_t)n0 * ldbb);
+    const int64_t offa = pa ? (int64_t)(m0 / 256) * panel : (int64_t)m0 * ldab;
+    const int64_t offb = pb ? (int64_t)(n0 / 256) * panel : (int64_t)n0 * ldbb;
+    rA[0] = raw_rsrc(static_cast<const char*>(resolve(src, S.A)) + offa);
+    rB[0] = raw_rsrc(static_cast<const char*>(resolve(src, S.B)) + offb);
     if constexpr (NPART == 2) {
-      rA[1] = raw_rsrc(static_cast<const char*>(resolve(src, S.Alo)) + (int64_t)m0 * ldab);
-      rB[1] = raw_rsrc(static_cast<const char*>(resolve(src, S.Blo)) + (int64_t)n0 * ldbb);
+      rA[1] = raw_rsrc(static_cast<const char*>(resolve(src, S.Alo)) + offa);
+      rB[1] = raw_rsrc(static_cast<const char*>(resolve(src, S.Blo)) + offb);
     }
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      vA[c] = m0 + crow[c] < P.M ? (unsigned)(crow[c] * ldab + cq * 16) : OOB;
-      vB[c] = n0 + crow[c] < P.N ? (unsigned)(crow[c] * ldbb + cq * 16) : OOB;
+      vA[c] = pa ? (unsigned)((threadIdx.x + c * BNT) * 16)
+                 : (m0 + crow[c] < P.M ? (unsigned)(crow[c] * ldab + cq * 16) : OOB);
+      vB[c] = pb ? (unsigned)((threadIdx.x + c * BNT) * 16)
+                 : (n0 + crow[c] < P.N ? (unsigned)(crow[c] * ldbb + cq * 16) : OOB);
     }
+    kmulA = pa ? 256u : 1u;
+    kmulB = pb ? 256u : 1u;
     Kb = S.K * EB;
     kend = S.K;
   };
@@ -445,8 +456,8 @@ __global__ __launch_bounds__(BNT) void gemm_mixed_big2_kernel(Src src) {
     const bool kin = kb0 + cq * 16 < Kb;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const unsigned oa = kin ? vA[c] + kb0 : OOB;
-      const unsigned ob = kin ? vB[c] + kb0 : OOB;
+      const unsigned oa = kin ? vA[c] + (unsigned)kb0 * kmulA : OOB;
+      const unsigned ob = kin ? vB[c] + (unsigned)kb0 * kmulB : OOB;
 #pragma unroll
       for (int p = 0; p < NPART; ++p) {
         ra[p][c] = __builtin_amdgcn_raw_buffer_load_b128(rA[p], (int)oa, 0, 0);
@@ -711,7 +722,11 @@ mk_status launch_mixed(const ProbSpec* probs, int nprob, hipStream_t stream) {
       gs.lda = q.lda; gs.ldb = q.ldb;
       // K rounded up to the padded row: the padding is zero
       gs.K = (int32_t)((q.K + 7) / 8 * 8);
-      gs.vec = 1;
+      // bit 0: vector loads; bits 1, 2: A, B in the packed panel layout
+      gs.vec = 1 | (q.packA ? 2 : 0) | (q.packB ? 4 : 0);
+      if (q.packA || q.packB)
+        MK_REQUIRE(VAR == V_BIG2 && ps.M % 256 == 0 && ps.N % 256 == 0 && q.K % 32 == 0, MK_BAD_ARG,
+                   "mixed gemm: packed operands need the 256 x 256 kernel, M, N multiples of 256 and K of 32");
     }
     P.tiles_m = (int32_t)cdiv(ps.M, BM);
     P.tiles_n = (int32_t)cdiv(ps.N, BN);
@@ -765,6 +780,14 @@ mk_status dispatch(const ProbSpec* probs, int nprob, hipStream_t stream) {
               probs[i].N * probs[i].seg[s].ldb * EB < (1ll << 31);
   }
   const int var = variant();
+  bool packed = false;
+  for (int i = 0; i < nprob; ++i)
+    for (int s = 0; s < probs[i].nseg; ++s) packed = packed || probs[i].seg[s].packA || probs[i].seg[s].packB;
+  if (packed) {
+    MK_REQUIRE(var == V_BIG2 && tiles256 >= 256 && rows256_fit<MODE>(probs, nprob), MK_BAD_ARG,
+               "mixed gemm: packed operands are read by the 256 x 256 kernel only");
+    return launch_mixed<256, 256, MODE, V_BIG2>(probs, nprob, stream);
+  }
   // large tiles once they alone fill the 256 CUs; 128x128 while they fill them; else 64x64
   if (var == V_PIPE && tilesp >= 256 && off32) return launch_mixed<PBM, PBN, MODE, V_PIPE>(probs, nprob, stream);
   if (var == V_BIG2 && tiles256 >= 256 && rows256_fit<MODE>(probs, nprob))
@@ -813,6 +836,14 @@ mk_status gemm_mixed_group(const GemmGroup& g, int prec, int kind, int grid, hip
   MK_REQUIRE(kind >= TK_64 && kind <= TK_256, MK_BAD_ARG, "gemm_mixed_group: tile kind %d", kind);
   if (prec == MK_BF16X3) return launch_group<M_BF16X3>(g, kind, grid, st);
   return launch_group<M_F32>(g, kind, grid, st);
+}
+
+bool mixed_packed_ok(int64_t M, int64_t N, const int64_t* Ks, int nk) {
+  if (variant() != V_BIG2 || M % 256 || N % 256 || (M / 256) * (N / 256) < 256) return false;
+  for (int i = 0; i < nk; ++i)
+    // a panel (256 rows x K, fp32 the wider) must stay within 32-bit buffer offsets
+    if (Ks[i] % 32 || Ks[i] * 4 * 256 >= (1ll << 31)) return false;
+  return true;
 }
 
 mk_status gemm_bf16x3(const ProbSpec* probs, int nprob, hipStream_t stream) {

@@ -109,10 +109,15 @@ mk_status k_to_f32(const double* M, int64_t rows, int64_t cols, int64_t ld, int 
 // mixed-precision nearest-neighbour apply: y = alpha CB(x) + beta y and the
 // K-contiguous conversions of x as left factor (xa, rows of length ka >= N) and
 // as right factor (xb, rows of length kb >= M); split = 1: bf16 hi/lo, 0: fp32
-// (xa_hi / xb_hi only).  Row padding is not written.
+// (xa_hi / xb_hi only).  Row padding is not written.  packed = 1: both views
+// in the packed panel layout of the 256 x 256 mixed kernel (ka = N, kb = M;
+// 2 Dl and 2 Dr multiples of 256).
 mk_status k_nn_prologue(const double* P, const double* x, int64_t Dl, int64_t Dr, double alpha, double beta,
                         double* y, int split, void* xa_hi, void* xa_lo, int64_t ka, void* xb_hi, void* xb_lo,
-                        int64_t kb, hipStream_t st);
+                        int64_t kb, int packed, hipStream_t st);
+// Packed panel layout (packed_offset) of an operand held as MN K-contiguous
+// rows of length kpad >= K, element size eb (2 or 4 bytes).
+mk_status k_repack(const void* in, int64_t MN, int64_t K, int64_t kpad, int eb, void* out, hipStream_t st);
 
 
 // ---- Jordan operator matrices from the constructed terms ------------------

@@ -605,7 +605,7 @@ __global__ void f32_trans_kernel(const double* __restrict__ M, int64_t rows, int
   }
 }
 
-template <bool SPLIT>
+template <bool SPLIT, bool PACK>
 __global__ __launch_bounds__(256) void nn_prologue_kernel(const double* __restrict__ P,
                                                           const double* __restrict__ x, int64_t Dl, int64_t Dr,
                                                           double alpha, double beta, double* __restrict__ y,
@@ -613,10 +613,23 @@ __global__ __launch_bounds__(256) void nn_prologue_kernel(const double* __restri
                                                           int64_t ka, void* __restrict__ xb_hi,
                                                           void* __restrict__ xb_lo, int64_t kb) {
   __shared__ ProTile t;
-  nn_prologue_tile<SPLIT>(P, x, Dl, Dr, alpha, beta, y, xa_hi, xa_lo, ka, xb_hi, xb_lo, kb,
-                          (int64_t)blockIdx.x * PRO_A, (int64_t)blockIdx.y * PRO_C, t);
+  nn_prologue_tile<SPLIT, PACK>(P, x, Dl, Dr, alpha, beta, y, xa_hi, xa_lo, ka, xb_hi, xb_lo, kb,
+                                (int64_t)blockIdx.x * PRO_A, (int64_t)blockIdx.y * PRO_C, t);
 }
 
+// Packed panel layout from K-contiguous rows (row length kpad >= K, element
+// size eb): one 16-byte chunk per thread.
+__global__ void repack_kernel(const char* __restrict__ in, int64_t MN, int64_t K, int64_t kpad, int eb,
+                              char* __restrict__ out) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) u4 gu4;
+  const int64_t steps = K * eb / 64;
+  GRID_STRIDE(o, MN * K * eb / 16) {
+    const int64_t slab = o / 1024, r = (o % 1024) / 4, q = o % 4;
+    const int64_t row = (slab / steps) * 256 + r, kbyte = (slab % steps) * 64 + q * 16;
+    ((gu4*)out)[o] = ((const gu4*)(in + row * kpad * eb + kbyte))[0];
+  }
+}
 
 // ---- Jordan operator matrices from constructed terms (prepare_operator!! folds)
 // Absent terms are null pointers; flags say which optional terms a target takes.
@@ -915,17 +928,36 @@ mk_status k_to_f32(const double* M, int64_t rows, int64_t cols, int64_t ld, int 
 
 mk_status k_nn_prologue(const double* P, const double* x, int64_t Dl, int64_t Dr, double alpha, double beta,
                         double* y, int split, void* xa_hi, void* xa_lo, int64_t ka, void* xb_hi, void* xb_lo,
-                        int64_t kb, hipStream_t st) {
+                        int64_t kb, int packed, hipStream_t st) {
   if (Dl * Dr == 0) return MK_OK;
   dim3 g((unsigned)cdiv(Dl, (int64_t)PRO_A), (unsigned)cdiv(Dr, (int64_t)PRO_C));
-  if (split)
-    hipLaunchKernelGGL(nn_prologue_kernel<true>, g, dim3(256), 0, st, P, x, Dl, Dr, alpha, beta, y, xa_hi, xa_lo, ka,
-                       xb_hi, xb_lo, kb);
+  if (packed)
+    MK_REQUIRE((2 * Dl) % 256 == 0 && (2 * Dr) % 256 == 0 && ka == 2 * Dr && kb == 2 * Dl, MK_BAD_ARG,
+               "nn_prologue: packed views need 2 Dl, 2 Dr multiples of 256 and unpadded rows");
+  if (split && packed)
+    hipLaunchKernelGGL((nn_prologue_kernel<true, true>), g, dim3(256), 0, st, P, x, Dl, Dr, alpha, beta, y, xa_hi,
+                       xa_lo, ka, xb_hi, xb_lo, kb);
+  else if (split)
+    hipLaunchKernelGGL((nn_prologue_kernel<true, false>), g, dim3(256), 0, st, P, x, Dl, Dr, alpha, beta, y, xa_hi,
+                       xa_lo, ka, xb_hi, xb_lo, kb);
+  else if (packed)
+    hipLaunchKernelGGL((nn_prologue_kernel<false, true>), g, dim3(256), 0, st, P, x, Dl, Dr, alpha, beta, y, xa_hi,
+                       nullptr, ka, xb_hi, nullptr, kb);
   else
-    hipLaunchKernelGGL(nn_prologue_kernel<false>, g, dim3(256), 0, st, P, x, Dl, Dr, alpha, beta, y, xa_hi, nullptr,
-                       ka, xb_hi, nullptr, kb);
+    hipLaunchKernelGGL((nn_prologue_kernel<false, false>), g, dim3(256), 0, st, P, x, Dl, Dr, alpha, beta, y, xa_hi,
+                       nullptr, ka, xb_hi, nullptr, kb);
   MK_CHECK_HIP(hipGetLastError());
   return MK_OK;
+}
+
+mk_status k_repack(const void* in, int64_t MN, int64_t K, int64_t kpad, int eb, void* out, hipStream_t st) {
+  MK_REQUIRE(MN % 256 == 0 && (K * eb) % 64 == 0 && kpad >= K && (kpad * eb) % 16 == 0 &&
+                 (uintptr_t)in % 16 == 0 && (uintptr_t)out % 16 == 0,
+             MK_BAD_ARG, "repack: rows %lld, K %lld, kpad %lld do not fit the packed layout", (long long)MN,
+             (long long)K, (long long)kpad);
+  if (MN * K == 0) return MK_OK;
+  LAUNCH_1D(repack_kernel, MN * K * eb / 16, st, static_cast<const char*>(in), MN, K, kpad, eb,
+            static_cast<char*>(out));
 }
 
 

@@ -95,6 +95,25 @@ mk_status check_jordan(const Wh& W, const char* name) {
   return MK_OK;
 }
 
+// The mixed two-site apply of a nearest-neighbour Jordan operator (d = 2, x*IC +
+// BE*x plus the physical term) converts x and computes the physical term in one
+// pass over x.
+// Packed operands for the fused apply on the 256 x 256 kernel; MK_PACKED=0
+// keeps the K-contiguous rows (read once).
+bool packing_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("MK_PACKED");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+bool nn_fused(const Op* op) {
+  const int64_t M = op->Dl * op->d1, N = op->Dr * op->d2;
+  return op->hasP && op->IC.present() && op->BE.present() && !op->CA.present() && op->AB.empty() &&
+         op->d1 == 2 && op->d2 == 2 && M % 8 == 0 && N % 8 == 0;
+}
+
 mk_status make_all_mixed(Op* op) {
   hipStream_t st = op->ctx->stream;
   const int pm = op->precisions & (MK_PREC_BF16X3 | MK_PREC_FP32);
@@ -103,6 +122,16 @@ mk_status make_all_mixed(Op* op) {
   for (OpMat* m : mats)
     if (m->present()) MK_TRY(m->make_mixed(pm, st));
   for (OpMat& m : op->AB) MK_TRY(m.make_mixed(pm, st));
+  if (op->kind == OP_AC2_JOR && nn_fused(op) && packing_enabled()) {
+    const int64_t M = op->Dl * op->d1, N = op->Dr * op->d2;
+    const int64_t Ks[2] = {N, M};
+    // BF16x3 only: the FP32 kernel measured 2% slower on packed operands
+    if ((pm & MK_PREC_BF16X3) && mixed_packed_ok(M, N, Ks, 2)) {
+      MK_TRY(op->IC.make_packed(MK_PREC_BF16X3, st));
+      MK_TRY(op->BE.make_packed(MK_PREC_BF16X3, st));
+      op->nn_packed = true;
+    }
+  }
   MK_CHECK_HIP(hipStreamSynchronize(st));
   return MK_OK;
 }
@@ -197,6 +226,12 @@ Side mat_side(const OpMat& m, int prec) {
   return {m.f32.ptr, nullptr, m.kpad};
 }
 Side view_side(const MixedView& v) { return {v.hi, v.lo, v.kpad}; }
+// The packed copies (OpMat::make_packed); the leading dimension is the K length.
+Side packed_side(const OpMat& m, int prec) {
+  const int64_t K = m.role == 'B' ? m.rows : m.cols;
+  if (prec == MK_BF16X3) return {m.phi.ptr, m.plo.ptr, K};
+  return {m.pf32.ptr, nullptr, K};
+}
 
 }  // namespace
 
@@ -264,7 +299,7 @@ mk_status apply_blocks(Op* op, const double* x, double* y, double alpha, double 
     const MixedView xa = view_layout(M, N, 'A', prec, mv);
     const MixedView xb = view_layout(M, N, 'B', prec, static_cast<char*>(mv) + va);
     MK_TRY(k_nn_prologue(static_cast<const double*>(op->P.ptr), x, Dl, Dr, 1.0, b0, out, prec == MK_BF16X3, xa.hi,
-                         xa.lo, xa.kpad, xb.hi, xb.lo, xb.kpad, st));
+                         xa.lo, xa.kpad, xb.hi, xb.lo, xb.kpad, 0, st));
   } else if (op->hasP) {
     MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, 1.0, b0, out, st));
   } else {
@@ -440,8 +475,9 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
   MixedView xa, xb, xca;
   // nearest-neighbour chains with d = 2: the CB term and both conversions of x
   // come from one pass over x
-  const bool fused = prec != MK_FP64 && op->hasP && op->IC.present() && op->BE.present() && !op->CA.present() &&
-                     op->AB.empty() && d1 == 2 && d2 == 2 && M % 8 == 0 && N % 8 == 0;
+  const bool fused = prec != MK_FP64 && nn_fused(op);
+  // packed operands (views of x written packed by the prologue, IC / BE packed at prepare)
+  const bool packed = fused && op->nn_packed && prec == MK_BF16X3;
   if (prec != MK_FP64) {
     void *ma = nullptr, *mb = nullptr, *mc = nullptr;
     const bool need_a = op->IC.present(), need_b = op->BE.present() || !op->AB.empty();
@@ -452,6 +488,12 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
     if (fused) {
       xa = view_layout(M, N, 'A', prec, ma);
       xb = view_layout(M, N, 'B', prec, mb);
+      if (packed) {
+        // the same memory, packed: no row padding, so kpad is the K length
+        // (N for xa, M for xb); the lo parts stay where the row layout puts them
+        xa.kpad = N;
+        xb.kpad = M;
+      }
     } else {
       if (need_a) MK_TRY(convert_view(ctx, x, M, N, M, 'A', prec, &xa, ma));
       if (need_b) MK_TRY(convert_view(ctx, x, M, N, M, 'B', prec, &xb, mb));
@@ -462,7 +504,8 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
   // two-site CB term first: a memory-bound pass that reads x once
   if (fused)
     MK_TRY(k_nn_prologue(static_cast<const double*>(op->P.ptr), x, Dl, Dr, alpha, bs.next(), y,
-                         prec == MK_BF16X3, xa.hi, xa.lo, xa.kpad, xb.hi, xb.lo, xb.kpad, ctx->stream));
+                         prec == MK_BF16X3, xa.hi, xa.lo, xa.kpad, xb.hi, xb.lo, xb.kpad, packed ? 1 : 0,
+                         ctx->stream));
   else if (op->hasP)
     MK_TRY(k_phys2(static_cast<const double*>(op->P.ptr), x, Dl, (int)d1, Dr, (int)d2, alpha, bs.next(), y,
                    ctx->stream));
@@ -471,13 +514,15 @@ mk_status apply_ac2_jordan(Op* op, const double* x, double* y, double alpha, dou
     p.M = M; p.N = N; p.C = y; p.ldc = M; p.alpha = alpha; p.beta = bs.next();
     if (op->IC.present()) {
       Side a = prec == MK_FP64 ? Side{x, nullptr, M} : view_side(xa);
-      Side b = mat_side(op->IC, prec);
+      Side b = packed ? packed_side(op->IC, prec) : mat_side(op->IC, prec);
       p.add(a.p, a.ld, b.p, b.ld, N, a.lo, b.lo);
+      p.seg[p.nseg - 1].packA = p.seg[p.nseg - 1].packB = packed;
     }
     if (op->BE.present()) {
-      Side a = mat_side(op->BE, prec);
+      Side a = packed ? packed_side(op->BE, prec) : mat_side(op->BE, prec);
       Side b = prec == MK_FP64 ? Side{x, nullptr, M} : view_side(xb);
       p.add(a.p, a.ld, b.p, b.ld, M, a.lo, b.lo);
+      p.seg[p.nseg - 1].packA = p.seg[p.nseg - 1].packB = packed;
     }
     MK_TRY(run_gemm(&p, 1, prec, nn, ctx->stream));
   }

@@ -83,6 +83,11 @@ struct OpMat {
   }
   void view(char r, double* p, int64_t nr, int64_t nc) { role = r; rows = nr; cols = nc; fp = p; }
   mk_status make_mixed(int precisions, hipStream_t st);
+  // copies of the mixed operands in the packed panel layout of the 256 x 256
+  // mixed kernel (packed_offset), besides the K-contiguous rows above
+  DevBuf phi, plo, pf32;
+  bool packed = false;
+  mk_status make_packed(int precisions, hipStream_t st);
 };
 
 struct MixedView {
@@ -125,6 +130,9 @@ struct Op {
   std::vector<OpMat> AB;  // (e, j) blocks at index e + d2*j
   DevBuf P;               // two-site (or one-site) physical term, if not folded
   bool hasP = false;
+  // the fused nearest-neighbour BF16x3 apply runs on the 256 x 256 kernel with
+  // packed operands (IC, BE packed at prepare; the views of x packed per apply)
+  bool nn_packed = false;
 
   // Jordan one-site terms
   OpMat Cp, Bp;
