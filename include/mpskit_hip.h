@@ -219,6 +219,18 @@ mk_status mk_transfer_right_mpo_levels(mk_ctx ctx, const mk_env_levels* v, const
 mk_status mk_krylov_orthogonalize(mk_ctx ctx, const double* V, int64_t ldv, int k, double* u, int64_t n,
                                   double* h);
 
+/* Cholesky factorisation A = L L^T of an n x n symmetric positive definite
+ * float64 matrix (column-major, leading dim lda, device pointer), in place:
+ * the lower triangle becomes L and the strict upper triangle zero.  *info
+ * (host) is 0 on success, else the 1-based column of the first non-positive
+ * pivot (then A holds a partial result).  The factorisation inside the
+ * Cholesky QR of the gauge steps' positive QR / LQ (qr_compact / lq_compact
+ * with positive = true in ortho.jl:215-340 and the finite-MPS gauge moves of
+ * finitemps.jl; the reference factors by Householder QR in LAPACK, which this
+ * library replaces by shifted Cholesky QR3); no counterpart entry point in the
+ * reference. */
+mk_status mk_potrf(mk_ctx ctx, double* A, int64_t n, int64_t lda, int* info);
+
 /* ---- U(1) charge blocks ------------------------------------------------------
  * Counterpart of TensorKit's block-sparse U(1) storage under the same
  * derivative operators (the reference applies one GEMM per fusion-tree block,

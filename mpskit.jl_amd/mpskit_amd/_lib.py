@@ -98,6 +98,7 @@ _SIGS = {
     "mk_op_compact_charge_blocks": ([_P], _INT),
     "mk_op_charge_blocks": ([_P, C.POINTER(_I64), _INT, C.POINTER(_INT)], _INT),
     "mk_krylov_orthogonalize": ([_P, _P, _I64, _INT, _P, _I64, C.POINTER(_D)], _INT),
+    "mk_potrf": ([_P, _P, _I64, _I64, C.POINTER(_INT)], _INT),
     "mk_apply": ([_P, _P, _P, _D, _D, _INT], _INT),
     "mk_apply_noalloc": ([_P, _P, _P, _D, _D, _INT], _INT),
     "mk_op_info": ([_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_D)], _INT),

@@ -41,3 +41,21 @@ def cgs2(Vb: torch.Tensor, k: int, w: torch.Tensor) -> np.ndarray:
     check(load().mk_krylov_orthogonalize(ctx.handle, C.c_void_p(Vb.data_ptr()), int(Vb.shape[1]), int(k),
                                          C.c_void_p(w.data_ptr()), int(w.numel()), h))
     return np.frombuffer(h, dtype=np.float64, count=k + 1).copy()
+
+
+def potrf_(G: torch.Tensor) -> tuple[torch.Tensor, int]:
+    """Lower Cholesky factor of a symmetric positive definite float64 GPU matrix,
+    computed in place in G's memory by the library (``mk_potrf``).  G's storage
+    is read as a column-major matrix, which for a symmetric G is G itself.
+    Returns (L, info) as ``torch.linalg.cholesky_ex`` does: L a view of G's
+    memory (its strict upper triangle zero), info 0 on success, else the 1-based
+    column of the first non-positive pivot."""
+    n = G.shape[0]
+    if not (G.is_cuda and G.dtype == torch.float64 and G.dim() == 2 and G.shape[1] == n and G.is_contiguous()):
+        raise ValueError("potrf_: needs a contiguous square float64 GPU matrix")
+    ctx = default_context()
+    ctx.sync_stream()
+    info = C.c_int(0)
+    check(load().mk_potrf(ctx.handle, C.c_void_p(G.data_ptr()), int(n), int(max(n, 1)), C.byref(info)))
+    # the column-major factor L in row-major storage reads as L^T
+    return G.t(), int(info.value)

@@ -52,12 +52,16 @@ def _householder_pos_qr(M):
 
 
 def _chol_step(Q, shift=0.0):
-    """One Cholesky QR pass: G = Q^T Q (+ shift I) = R^T R, Q <- Q R^-1."""
-    G = Q.t() @ Q
+    """One Cholesky QR pass: G = Q^T Q (+ shift I) = R^T R, Q <- Q R^-1.  The
+    factorisation is the library's blocked Cholesky (``mk_potrf``)."""
+    G = (Q.t() @ Q).contiguous()
     if shift:
         G.diagonal().add_(shift)
-    L, info = torch.linalg.cholesky_ex(G)
-    return L, info
+    if not G.is_cuda:
+        # host tensors (the CPU tests of this QR's logic)
+        return torch.linalg.cholesky_ex(G)
+    from .krylov import potrf_
+    return potrf_(G)
 
 
 def pos_qr(M):
