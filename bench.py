@@ -540,13 +540,44 @@ def vumps_workload(args):
     """VUMPS on the infinite J1-J2 chain (J2/J1 = 0.24, unit cell 2, w = 8) at bond
     dimension D: seconds per iteration in float64 and in the mixed path (FP64
     environment normalisation, gauge and energies in both), over a fixed number
-    of iterations from the same random state."""
+    of iterations from the same random state, with the time of each part of the
+    iterations (host timers, device synchronised)."""
+    import collections
     import torch
     import mpskit_amd as mk
     from mpskit_amd import vumps as V
+    der = sys.modules["mpskit_amd.derivatives"]
     D, iters = args.vumps_D, args.vumps_iters
+    # host timers (device synchronised) around the parts of an iteration
+    acc, cnt = collections.defaultdict(float), collections.defaultdict(int)
+
+    def timed(obj, name, key_of):
+        f = getattr(obj, name)
+
+        def g(*a, **k):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = f(*a, **k)
+            torch.cuda.synchronize()
+            key = key_of(a)
+            acc[key] += time.perf_counter() - t
+            cnt[key] += 1
+            return r
+        setattr(obj, name, g)
+
+    timed(der.PreparedDerivative, "eigsolve", lambda a: "AC eigensolves" if len(a[0].x_shape) == 3 else "C eigensolves")
+    timed(V, "uniform_rightorth", lambda a: "gauge (right orthogonalisation of the cell)")
+    # inside the gauge step (also counted in it): its LQ factorisations and fixed-point Arnoldi
+    timed(V, "lq_right", lambda a: "  gauge: positive LQ factorisations")
+    timed(V, "arnoldi_dominant", lambda a: "  gauge: fixed-point Arnoldi")
+    timed(V.InfiniteEnvironments, "recalculate", lambda a: "environments")
+    timed(V, "regauge", lambda a: "regauge")
+    timed(V, "calc_galerkin", lambda a: "Galerkin error")
+    timed(V, "prepare_operator", lambda a: "operator preparation")
     res = {}
     for prec in ("fp64", "bf16x3"):
+        acc.clear()
+        cnt.clear()
         H = V.InfiniteHamiltonian([mk.models.j1j2_bulk(1.0, 0.24)] * 2)
         psi = V.InfiniteMPS.random(2, 2, D, seed=7)
         print(f"# vumps {prec} D={D}", file=sys.stderr, flush=True)
@@ -557,7 +588,8 @@ def vumps_workload(args):
         res[prec] = dict(seconds_per_iteration=sum(per) / len(per), energy_per_site=hist[-1]["energy"],
                          galerkin=hist[-1]["eps"], iterations=len(hist), total_seconds=sum(h["time"] for h in hist),
                          seconds_per_iteration_last5=sum(last) / len(last),
-                         energies=[h["energy"] for h in hist], errors=[h["eps"] for h in hist])
+                         energies=[h["energy"] for h in hist], errors=[h["eps"] for h in hist],
+                         parts={k: {"seconds": acc[k], "calls": cnt[k]} for k in sorted(acc, key=lambda k: -acc[k])})
     out = {
         "metric": "VUMPS time per iteration, infinite J1-J2 chain (J2/J1=0.24, cell 2, w=8)",
         "value": res["bf16x3"]["seconds_per_iteration"], "unit": "s/iteration", "n_gpus": 1, "steps": iters,
