@@ -331,6 +331,60 @@ mk_status mk_transfer_right_mpo(mk_ctx ctx, const double* v, const double* O,
                                 int64_t d, int64_t Di, int64_t wl, int64_t wr,
                                 int64_t Dc, int64_t Do, double* y);
 
+/* Quasiparticle transfers and transfers with legs passing through.  A
+ * quasiparticle (excited) tensor A carries an excitation leg e between its
+ * physical leg and its right bond, A (Dn x d x De x Dc) as the reference's
+ * A[l p; e r].  O = NULL stands for the identity MPO on the passenger leg of
+ * x / v (then wl == wr; wl == 1 is the MPS-only transfer).  All FP64, on the
+ * context stream; braidings are trivial (bosonic spaces).
+ *
+ * transfer_left with an excited upper tensor: transfer.jl:58-60 (x a bond
+ * tensor: wl = 1, O = NULL), :66-69 (x with a passing leg g: wl = Dg,
+ * O = NULL), :133-135 (MPO transfer):
+ *   y[a,b,e,c] = x[m,g,n] A[n,h,e,c] O[g,k,h,b] Abar[m,k,a];
+ *   x (Dm x wl x Dn), O (wl x d x d x wr), Abar (Dm x d x Da) -> y (Da x wr x De x Dc). */
+mk_status mk_transfer_left_excited(mk_ctx ctx, const double* x, const double* O, const double* A,
+                                   const double* Abar, int64_t Dm, int64_t wl, int64_t Dn, int64_t d,
+                                   int64_t De, int64_t wr, int64_t Dc, int64_t Da, double* y);
+
+/* transfer_right with an excited upper tensor: transfer.jl:61-63, :70-73,
+ * :136-138:
+ *   y[a,b,e,c] = A[a,h,e,i] O[b,k,h,l] Abar[c,k,o] v[i,l,o];
+ *   A (Da x d x De x Di), O (wl x d x d x wr), Abar (Db x d x Dj),
+ *   v (Di x wr x Dj) -> y (Da x wl x De x Db). */
+mk_status mk_transfer_right_excited(mk_ctx ctx, const double* v, const double* O, const double* A,
+                                    const double* Abar, int64_t Da, int64_t d, int64_t De, int64_t Di,
+                                    int64_t wl, int64_t wr, int64_t Db, int64_t Dj, double* y);
+
+/* transfer_left of an environment with an excitation (or utility) leg e in
+ * its domain: transfer.jl:101-104 (O = NULL), :141-143 (MPO transfer; the
+ * left quasiparticle environments, src/environments/qp_envs.jl:149-153):
+ *   y[a,b,e,c] = x[m,g,e,n] A[n,h,c] O[g,k,h,b] Abar[m,k,a];
+ *   x (Dm x wl x De x Dn), A (Dn x d x Dc), Abar (Dm x d x Da) -> y (Da x wr x De x Dc).
+ * With De = 1 and O = NULL this is also the transfer of a density matrix
+ * with a utility leg (transfer.jl:76-79). */
+mk_status mk_transfer_left_leg(mk_ctx ctx, const double* x, const double* O, const double* A,
+                               const double* Abar, int64_t Dm, int64_t wl, int64_t De, int64_t Dn,
+                               int64_t d, int64_t wr, int64_t Dc, int64_t Da, double* y);
+
+/* transfer_right counterpart: transfer.jl:105-108, :144-148 (qp_envs.jl:156-160):
+ *   y[a,b,e,c] = A[a,h,i] O[b,k,h,l] Abar[c,k,o] v[i,l,e,o];
+ *   A (Da x d x Di), Abar (Db x d x Dj), v (Di x wr x De x Dj) -> y (Da x wl x De x Db).
+ * De = 1, O = NULL: transfer.jl:80-83. */
+mk_status mk_transfer_right_leg(mk_ctx ctx, const double* v, const double* O, const double* A,
+                                const double* Abar, int64_t Da, int64_t d, int64_t Di, int64_t wl,
+                                int64_t wr, int64_t De, int64_t Db, int64_t Dj, double* y);
+
+/* An MPO tensor widened by identities on a virtual leg (ne) and a physical
+ * leg (nq): out[(w,e),(k,q),(h,r),(v,f)] = O[w,k,h,v] delta(e,f) delta(q,r),
+ * out ((wl ne) x (d nq) x (d nq) x (wr ne)), fused indices w + wl e and
+ * k + d q; O = NULL is the identity MPO (wl == wr).  With nq > 1 it turns the
+ * density-matrix transfers of transfer.jl:150-161 (an MPO acting on
+ * the first of two physical legs) into mk_transfer_left/right_mpo over the
+ * fused physical leg. */
+mk_status mk_mpo_extend(mk_ctx ctx, const double* O, int64_t wl, int64_t d, int64_t wr, int64_t ne,
+                        int64_t nq, double* out);
+
 /* A transfer matrix prepared for repeated applies (a TransferMatrix of the
  * reference, src/transfermatrix/transfermatrix.jl:4-33, applied by the
  * fixed-point solvers of src/environments/infinite_envs.jl:207-346 and

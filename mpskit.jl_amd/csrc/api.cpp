@@ -248,15 +248,13 @@ extern "C" mk_status mk_ctx_synchronize(mk_ctx h) {
 
 static bool pos(int64_t v) { return v > 0 && v < (1ll << 31); }
 
-extern "C" mk_status mk_transfer_left_mps(mk_ctx h, const double* v, const double* A,
-                                          const double* Ab, int64_t Dm, int64_t Dn, int64_t d,
-                                          int64_t Dc, int64_t Da, double* y) {
-  CTX_OR_FAIL(h);
-  MK_REQUIRE(pos(Dm) && pos(Dn) && pos(d) && pos(Dc) && pos(Da), MK_BAD_SHAPE,
-             "transfer_left: non-positive dimension");
-  MK_REQUIRE(v && A && Ab && y, MK_BAD_ARG, "transfer_left: null pointer");
-  double* t = nullptr;
-  MK_TRY(carve(ctx, {{(size_t)Dm * d * Dc * 8, (void**)&t}}));
+// The four FP64 transfers on caller-given scratch (the entry points below
+// carve it from the context arena in one piece, so a composed transfer never
+// hands an inner step a region that overlaps its own).
+
+// y[a,c] = v[m,n] A[n,h,c] Ab[m,h,a]; t holds Dm d Dc doubles
+static mk_status tl_mps_run(Ctx* ctx, const double* v, const double* A, const double* Ab, int64_t Dm, int64_t Dn,
+                            int64_t d, int64_t Dc, int64_t Da, double* y, double* t) {
   // t[m,(h,c)] = v[m,n] A[n,(h,c)]
   ProbSpec p1;
   p1.M = Dm; p1.N = d * Dc; p1.C = t; p1.ldc = Dm;
@@ -269,15 +267,9 @@ extern "C" mk_status mk_transfer_left_mps(mk_ctx h, const double* v, const doubl
   return gemm_f64(&p2, 1, GemmShape{true, false}, ctx->stream);
 }
 
-extern "C" mk_status mk_transfer_right_mps(mk_ctx h, const double* v, const double* A,
-                                           const double* Ab, int64_t Da, int64_t d, int64_t Di,
-                                           int64_t Db, int64_t Dj, double* y) {
-  CTX_OR_FAIL(h);
-  MK_REQUIRE(pos(Da) && pos(d) && pos(Di) && pos(Db) && pos(Dj), MK_BAD_SHAPE,
-             "transfer_right: non-positive dimension");
-  MK_REQUIRE(v && A && Ab && y, MK_BAD_ARG, "transfer_right: null pointer");
-  double* t = nullptr;
-  MK_TRY(carve(ctx, {{(size_t)Da * d * Dj * 8, (void**)&t}}));
+// y[a,b] = A[a,h,i] Ab[b,h,j] v[i,j]; t holds Da d Dj doubles
+static mk_status tr_mps_run(Ctx* ctx, const double* v, const double* A, const double* Ab, int64_t Da, int64_t d,
+                            int64_t Di, int64_t Db, int64_t Dj, double* y, double* t) {
   // t[(a,h),j] = A[(a,h),i] v[i,j]
   ProbSpec p1;
   p1.M = Da * d; p1.N = Dj; p1.C = t; p1.ldc = Da * d;
@@ -290,16 +282,11 @@ extern "C" mk_status mk_transfer_right_mps(mk_ctx h, const double* v, const doub
   return gemm_f64(&p2, 1, GemmShape{false, true}, ctx->stream);
 }
 
-extern "C" mk_status mk_transfer_left_mpo(mk_ctx h, const double* x, const double* O,
-                                          const double* A, const double* Ab, int64_t Dm, int64_t wl,
-                                          int64_t Dn, int64_t d, int64_t wr, int64_t Dc, int64_t Da,
-                                          double* y) {
-  CTX_OR_FAIL(h);
-  MK_REQUIRE(pos(Dm) && pos(wl) && pos(Dn) && pos(d) && pos(wr) && pos(Dc) && pos(Da), MK_BAD_SHAPE,
-             "transfer_left_mpo: non-positive dimension");
-  MK_REQUIRE(x && O && A && Ab && y, MK_BAD_ARG, "transfer_left_mpo: null pointer");
-  double *t1 = nullptr, *t2 = nullptr;
-  MK_TRY(carve(ctx, {{(size_t)Dm * wl * d * Dc * 8, (void**)&t1}, {(size_t)Dm * d * wr * Dc * 8, (void**)&t2}}));
+// y[a,b,c] = x[m,g,n] A[n,h,c] O[g,k,h,b] Ab[m,k,a]; t1 holds Dm wl d Dc and
+// t2 Dm d wr Dc doubles
+static mk_status tl_mpo_run(Ctx* ctx, const double* x, const double* O, const double* A, const double* Ab, int64_t Dm,
+                            int64_t wl, int64_t Dn, int64_t d, int64_t wr, int64_t Dc, int64_t Da, double* y,
+                            double* t1, double* t2) {
   // t1[(m,g),(h,c)] = x[(m,g),n] A[n,(h,c)]
   ProbSpec p1;
   p1.M = Dm * wl; p1.N = d * Dc; p1.C = t1; p1.ldc = Dm * wl;
@@ -313,16 +300,11 @@ extern "C" mk_status mk_transfer_left_mpo(mk_ctx h, const double* x, const doubl
   return gemm_f64(&p2, 1, GemmShape{true, false}, ctx->stream);
 }
 
-extern "C" mk_status mk_transfer_right_mpo(mk_ctx h, const double* v, const double* O,
-                                           const double* A, const double* Ab, int64_t Da, int64_t d,
-                                           int64_t Di, int64_t wl, int64_t wr, int64_t Dc, int64_t Do,
-                                           double* y) {
-  CTX_OR_FAIL(h);
-  MK_REQUIRE(pos(Da) && pos(d) && pos(Di) && pos(wl) && pos(wr) && pos(Dc) && pos(Do), MK_BAD_SHAPE,
-             "transfer_right_mpo: non-positive dimension");
-  MK_REQUIRE(v && O && A && Ab && y, MK_BAD_ARG, "transfer_right_mpo: null pointer");
-  double *t1 = nullptr, *t2 = nullptr;
-  MK_TRY(carve(ctx, {{(size_t)Da * d * wr * Do * 8, (void**)&t1}, {(size_t)Da * wl * d * Do * 8, (void**)&t2}}));
+// y[a,b,c] = A[a,h,i] O[b,k,h,l] Ab[c,k,o] v[i,l,o]; t1 holds Da d wr Do and
+// t2 Da wl d Do doubles
+static mk_status tr_mpo_run(Ctx* ctx, const double* v, const double* O, const double* A, const double* Ab, int64_t Da,
+                            int64_t d, int64_t Di, int64_t wl, int64_t wr, int64_t Dc, int64_t Do, double* y,
+                            double* t1, double* t2) {
   // t1[(a,h),(l,o)] = A[(a,h),i] v[i,(l,o)]
   ProbSpec p1;
   p1.M = Da * d; p1.N = wr * Do; p1.C = t1; p1.ldc = Da * d;
@@ -334,6 +316,177 @@ extern "C" mk_status mk_transfer_right_mpo(mk_ctx h, const double* v, const doub
   p2.M = Da * wl; p2.N = Dc; p2.C = y; p2.ldc = Da * wl;
   p2.add(t2, Da * wl, Ab, Dc, d * Do);
   return gemm_f64(&p2, 1, GemmShape{false, true}, ctx->stream);
+}
+
+static size_t f64b(int64_t n) { return (size_t)n * 8; }
+
+extern "C" mk_status mk_transfer_left_mps(mk_ctx h, const double* v, const double* A,
+                                          const double* Ab, int64_t Dm, int64_t Dn, int64_t d,
+                                          int64_t Dc, int64_t Da, double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Dm) && pos(Dn) && pos(d) && pos(Dc) && pos(Da), MK_BAD_SHAPE,
+             "transfer_left: non-positive dimension");
+  MK_REQUIRE(v && A && Ab && y, MK_BAD_ARG, "transfer_left: null pointer");
+  double* t = nullptr;
+  MK_TRY(carve(ctx, {{f64b(Dm * d * Dc), (void**)&t}}));
+  return tl_mps_run(ctx, v, A, Ab, Dm, Dn, d, Dc, Da, y, t);
+}
+
+extern "C" mk_status mk_transfer_right_mps(mk_ctx h, const double* v, const double* A,
+                                           const double* Ab, int64_t Da, int64_t d, int64_t Di,
+                                           int64_t Db, int64_t Dj, double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Da) && pos(d) && pos(Di) && pos(Db) && pos(Dj), MK_BAD_SHAPE,
+             "transfer_right: non-positive dimension");
+  MK_REQUIRE(v && A && Ab && y, MK_BAD_ARG, "transfer_right: null pointer");
+  double* t = nullptr;
+  MK_TRY(carve(ctx, {{f64b(Da * d * Dj), (void**)&t}}));
+  return tr_mps_run(ctx, v, A, Ab, Da, d, Di, Db, Dj, y, t);
+}
+
+extern "C" mk_status mk_transfer_left_mpo(mk_ctx h, const double* x, const double* O,
+                                          const double* A, const double* Ab, int64_t Dm, int64_t wl,
+                                          int64_t Dn, int64_t d, int64_t wr, int64_t Dc, int64_t Da,
+                                          double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Dm) && pos(wl) && pos(Dn) && pos(d) && pos(wr) && pos(Dc) && pos(Da), MK_BAD_SHAPE,
+             "transfer_left_mpo: non-positive dimension");
+  MK_REQUIRE(x && O && A && Ab && y, MK_BAD_ARG, "transfer_left_mpo: null pointer");
+  double *t1 = nullptr, *t2 = nullptr;
+  MK_TRY(carve(ctx, {{f64b(Dm * wl * d * Dc), (void**)&t1}, {f64b(Dm * d * wr * Dc), (void**)&t2}}));
+  return tl_mpo_run(ctx, x, O, A, Ab, Dm, wl, Dn, d, wr, Dc, Da, y, t1, t2);
+}
+
+extern "C" mk_status mk_transfer_right_mpo(mk_ctx h, const double* v, const double* O,
+                                           const double* A, const double* Ab, int64_t Da, int64_t d,
+                                           int64_t Di, int64_t wl, int64_t wr, int64_t Dc, int64_t Do,
+                                           double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Da) && pos(d) && pos(Di) && pos(wl) && pos(wr) && pos(Dc) && pos(Do), MK_BAD_SHAPE,
+             "transfer_right_mpo: non-positive dimension");
+  MK_REQUIRE(v && O && A && Ab && y, MK_BAD_ARG, "transfer_right_mpo: null pointer");
+  double *t1 = nullptr, *t2 = nullptr;
+  MK_TRY(carve(ctx, {{f64b(Da * d * wr * Do), (void**)&t1}, {f64b(Da * wl * d * Do), (void**)&t2}}));
+  return tr_mpo_run(ctx, v, O, A, Ab, Da, d, Di, wl, wr, Dc, Do, y, t1, t2);
+}
+
+// ---- quasiparticle and passenger-leg transfers ----------------------------
+// Every variant is one of the transfers above on reshaped operands: an
+// excitation leg e of A fuses with its right bond (left transfers) or, after
+// the two middle legs of A are exchanged, with its left bond (right
+// transfers); a leg that passes through x / v widens the MPO by an identity
+// on that leg (mpo_extend), and a missing MPO is the identity on the
+// passenger legs.
+
+static bool small_leg(int64_t a, int64_t b) { return a > 0 && b > 0 && a * b < (1 << 20); }
+
+extern "C" mk_status mk_mpo_extend(mk_ctx h, const double* O, int64_t wl, int64_t d, int64_t wr, int64_t ne,
+                                   int64_t nq, double* out) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(small_leg(wl, ne) && small_leg(wr, ne) && small_leg(d, nq), MK_BAD_SHAPE,
+             "mpo_extend: bad dimension (wl %lld, d %lld, wr %lld, ne %lld, nq %lld)", (long long)wl, (long long)d,
+             (long long)wr, (long long)ne, (long long)nq);
+  MK_REQUIRE(O != nullptr || wl == wr, MK_BAD_SHAPE, "mpo_extend: the identity MPO needs wl == wr");
+  MK_REQUIRE(out != nullptr, MK_BAD_ARG, "mpo_extend: null output");
+  return k_mpo_extend(O, (int)wl, (int)d, (int)wr, (int)ne, (int)nq, out, ctx->stream);
+}
+
+extern "C" mk_status mk_transfer_left_excited(mk_ctx h, const double* x, const double* O, const double* A,
+                                              const double* Ab, int64_t Dm, int64_t wl, int64_t Dn, int64_t d,
+                                              int64_t De, int64_t wr, int64_t Dc, int64_t Da, double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Dm) && small_leg(wl, 1) && pos(Dn) && small_leg(d, d) && small_leg(wr, 1) && pos(Dc) && pos(Da) &&
+                 pos(De) && pos(De * Dc),
+             MK_BAD_SHAPE, "transfer_left_excited: bad dimension");
+  MK_REQUIRE(x && A && Ab && y, MK_BAD_ARG, "transfer_left_excited: null pointer");
+  MK_REQUIRE(O != nullptr || wl == wr, MK_BAD_SHAPE, "transfer_left_excited: without an MPO wl must equal wr");
+  const int64_t Dc2 = De * Dc;   // A[n,h,(e,c)]
+  if (O == nullptr && wl == 1) {
+    double* t = nullptr;
+    MK_TRY(carve(ctx, {{f64b(Dm * d * Dc2), (void**)&t}}));
+    return tl_mps_run(ctx, x, A, Ab, Dm, Dn, d, Dc2, Da, y, t);
+  }
+  double *t1 = nullptr, *t2 = nullptr, *Oi = nullptr;
+  MK_TRY(carve(ctx, {{f64b(Dm * wl * d * Dc2), (void**)&t1},
+                     {f64b(Dm * d * wr * Dc2), (void**)&t2},
+                     {f64b(O ? 1 : wl * d * d * wl), (void**)&Oi}}));
+  if (O == nullptr) {
+    MK_TRY(k_mpo_extend(nullptr, (int)wl, (int)d, (int)wl, 1, 1, Oi, ctx->stream));
+    O = Oi;
+  }
+  return tl_mpo_run(ctx, x, O, A, Ab, Dm, wl, Dn, d, wr, Dc2, Da, y, t1, t2);
+}
+
+extern "C" mk_status mk_transfer_right_excited(mk_ctx h, const double* v, const double* O, const double* A,
+                                               const double* Ab, int64_t Da, int64_t d, int64_t De, int64_t Di,
+                                               int64_t wl, int64_t wr, int64_t Db, int64_t Dj, double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Da) && small_leg(d, d) && pos(De) && pos(Da * De) && pos(Di) && small_leg(wl, 1) &&
+                 small_leg(wr, 1) && pos(Db) && pos(Dj),
+             MK_BAD_SHAPE, "transfer_right_excited: bad dimension");
+  MK_REQUIRE(v && A && Ab && y, MK_BAD_ARG, "transfer_right_excited: null pointer");
+  MK_REQUIRE(O != nullptr || wl == wr, MK_BAD_SHAPE, "transfer_right_excited: without an MPO wl must equal wr");
+  const int64_t Da2 = Da * De;          // A'[(a,e),h,i]
+  const bool permA = De > 1;
+  const bool permY = De > 1 && wl > 1;  // the transfer yields y'[a,e,w,b]
+  const bool mps = O == nullptr && wl == 1;
+  double *Ap = nullptr, *yp = nullptr, *t1 = nullptr, *t2 = nullptr, *Oi = nullptr;
+  MK_TRY(carve(ctx, {{f64b(permA ? Da2 * d * Di : 1), (void**)&Ap},
+                     {f64b(permY ? Da2 * wl * Db : 1), (void**)&yp},
+                     {f64b(mps ? Da2 * d * Dj : Da2 * d * wr * Dj), (void**)&t1},
+                     {f64b(mps ? 1 : Da2 * wl * d * Dj), (void**)&t2},
+                     {f64b(O || mps ? 1 : wl * d * d * wl), (void**)&Oi}}));
+  const double* Au = A;
+  if (permA) {
+    MK_TRY(k_swap_mid(A, Da, d, De, Di, Ap, ctx->stream));
+    Au = Ap;
+  }
+  double* yo = permY ? yp : y;
+  if (mps) {
+    MK_TRY(tr_mps_run(ctx, v, Au, Ab, Da2, d, Di, Db, Dj, yo, t1));
+  } else {
+    if (O == nullptr) {
+      MK_TRY(k_mpo_extend(nullptr, (int)wl, (int)d, (int)wl, 1, 1, Oi, ctx->stream));
+      O = Oi;
+    }
+    MK_TRY(tr_mpo_run(ctx, v, O, Au, Ab, Da2, d, Di, wl, wr, Db, Dj, yo, t1, t2));
+  }
+  if (permY) MK_TRY(k_swap_mid(yp, Da, De, wl, Db, y, ctx->stream));
+  return MK_OK;
+}
+
+extern "C" mk_status mk_transfer_left_leg(mk_ctx h, const double* x, const double* O, const double* A,
+                                          const double* Ab, int64_t Dm, int64_t wl, int64_t De, int64_t Dn,
+                                          int64_t d, int64_t wr, int64_t Dc, int64_t Da, double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Dm) && small_leg(wl, De) && small_leg(wr, De) && pos(Dn) && small_leg(d, d) && pos(Dc) && pos(Da),
+             MK_BAD_SHAPE, "transfer_left_leg: bad dimension");
+  MK_REQUIRE(x && A && Ab && y, MK_BAD_ARG, "transfer_left_leg: null pointer");
+  MK_REQUIRE(O != nullptr || wl == wr, MK_BAD_SHAPE, "transfer_left_leg: without an MPO wl must equal wr");
+  const int64_t WL = wl * De, WR = wr * De;   // x[m,(g,e),n]
+  double *t1 = nullptr, *t2 = nullptr, *Oe = nullptr;
+  MK_TRY(carve(ctx, {{f64b(Dm * WL * d * Dc), (void**)&t1},
+                     {f64b(Dm * d * WR * Dc), (void**)&t2},
+                     {f64b(WL * d * d * WR), (void**)&Oe}}));
+  MK_TRY(k_mpo_extend(O, (int)wl, (int)d, (int)wr, (int)De, 1, Oe, ctx->stream));
+  return tl_mpo_run(ctx, x, Oe, A, Ab, Dm, WL, Dn, d, WR, Dc, Da, y, t1, t2);
+}
+
+extern "C" mk_status mk_transfer_right_leg(mk_ctx h, const double* v, const double* O, const double* A,
+                                           const double* Ab, int64_t Da, int64_t d, int64_t Di, int64_t wl,
+                                           int64_t wr, int64_t De, int64_t Db, int64_t Dj, double* y) {
+  CTX_OR_FAIL(h);
+  MK_REQUIRE(pos(Da) && small_leg(d, d) && pos(Di) && small_leg(wl, De) && small_leg(wr, De) && pos(Db) && pos(Dj),
+             MK_BAD_SHAPE, "transfer_right_leg: bad dimension");
+  MK_REQUIRE(v && A && Ab && y, MK_BAD_ARG, "transfer_right_leg: null pointer");
+  MK_REQUIRE(O != nullptr || wl == wr, MK_BAD_SHAPE, "transfer_right_leg: without an MPO wl must equal wr");
+  const int64_t WL = wl * De, WR = wr * De;   // v[i,(l,e),j]
+  double *t1 = nullptr, *t2 = nullptr, *Oe = nullptr;
+  MK_TRY(carve(ctx, {{f64b(Da * d * WR * Dj), (void**)&t1},
+                     {f64b(Da * WL * d * Dj), (void**)&t2},
+                     {f64b(WL * d * d * WR), (void**)&Oe}}));
+  MK_TRY(k_mpo_extend(O, (int)wl, (int)d, (int)wr, (int)De, 1, Oe, ctx->stream));
+  return tr_mpo_run(ctx, v, Oe, A, Ab, Da, d, Di, WL, WR, Db, Dj, y, t1, t2);
 }
 
 extern "C" mk_status mk_regularize(mk_ctx h, double* v, const double* lvec, const double* rvec,

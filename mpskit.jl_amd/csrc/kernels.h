@@ -48,6 +48,12 @@ mk_status k_tl_mid(const double* t1, const double* O, int64_t Dm, int wl, int d,
 // transfer-right middle step: t2[a,b,k,o] = sum_{h,l} t1[a,h,l,o] O[b,k,h,l]
 mk_status k_tr_mid(const double* t1, const double* O, int64_t Da, int wl, int d, int wr,
                    int64_t Do, double* t2, hipStream_t st);
+// out[a,q,p,b] = in[a,p,q,b] for in of shape n0 x n1 x n2 x n3
+mk_status k_swap_mid(const double* in, int64_t n0, int64_t n1, int64_t n2, int64_t n3, double* out,
+                     hipStream_t st);
+// out[(w,e),(k,q),(h,r),(v,f)] = O[w,k,h,v] delta(e,f) delta(q,r), shape
+// (wl ne) x (d nq) x (d nq) x (wr ne); O == nullptr is the identity MPO (wl == wr)
+mk_status k_mpo_extend(const double* O, int wl, int d, int wr, int ne, int nq, double* out, hipStream_t st);
 // The MPO steps above written as the K-contiguous mixed operand (bf16 hi/lo if
 // split, else fp32) of the transfer's second GEMM, row pitch ldo (zero padded).
 mk_status k_tl_mid_mixed(const double* t1, const double* O, int64_t Dm, int wl, int d, int wr, int64_t Dc,

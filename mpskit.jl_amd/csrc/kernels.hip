@@ -205,6 +205,45 @@ __global__ void tr_mid_kernel(const double* __restrict__ t1, const double* __res
   }
 }
 
+// out[a,q,p,b] = in[a,p,q,b]: the two middle legs of a column-major
+// n0 x n1 x n2 x n3 tensor exchanged (reads and writes run along a).
+__global__ void swap_mid_kernel(const double* __restrict__ in, int64_t n0, int64_t n1, int64_t n2, int64_t n3,
+                                double* __restrict__ out) {
+  const int64_t n = n0 * n1 * n2 * n3;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int64_t a = t % n0; t /= n0;
+    const int64_t q = t % n2; t /= n2;
+    const int64_t p = t % n1;
+    const int64_t b = t / n1;
+    out[idx] = in[a + n0 * (p + n1 * (q + n2 * b))];
+  }
+}
+
+// An MPO tensor widened by identities on a virtual and a physical passenger
+// leg: out[(w,e),(k,q),(h,r),(v,f)] = O[w,k,h,v] delta(e,f) delta(q,r) with
+// the fused indices w + wl e, k + d q (column-major fusion of the legs).
+// O == nullptr stands for the identity MPO O[w,k,h,v] = delta(w,v) delta(k,h)
+// (wl == wr).
+__global__ void mpo_extend_kernel(const double* __restrict__ O, int wl, int d, int wr, int ne, int nq,
+                                  double* __restrict__ out) {
+  const int64_t WL = (int64_t)wl * ne, DQ = (int64_t)d * nq, WR = (int64_t)wr * ne;
+  const int64_t n = WL * DQ * DQ * WR;
+  GRID_STRIDE(idx, n) {
+    int64_t t = idx;
+    const int i0 = (int)(t % WL); t /= WL;
+    const int i1 = (int)(t % DQ); t /= DQ;
+    const int i2 = (int)(t % DQ);
+    const int i3 = (int)(t / DQ);
+    const int w = i0 % wl, e = i0 / wl, k = i1 % d, q = i1 / d;
+    const int h = i2 % d, r = i2 / d, v = i3 % wr, f = i3 / wr;
+    double val = 0.0;
+    if (e == f && q == r)
+      val = O ? O[w + (int64_t)wl * (k + (int64_t)d * (h + (int64_t)d * v))] : (w == v && k == h ? 1.0 : 0.0);
+    out[idx] = val;
+  }
+}
+
 // The MPO step of the left transfer written straight into the K-contiguous
 // mixed-precision operand of the second GEMM (one pass instead of a float64
 // round trip and a conversion):
@@ -768,6 +807,13 @@ mk_status k_tl_mid(const double* t1, const double* O, int64_t Dm, int wl, int d,
 mk_status k_tr_mid(const double* t1, const double* O, int64_t Da, int wl, int d, int wr, int64_t Do,
                    double* t2, hipStream_t st) {
   LAUNCH_1D(tr_mid_kernel, Da * wl * d * Do, st, t1, O, Da, wl, d, wr, Do, t2);
+}
+mk_status k_swap_mid(const double* in, int64_t n0, int64_t n1, int64_t n2, int64_t n3, double* out,
+                     hipStream_t st) {
+  LAUNCH_1D(swap_mid_kernel, n0 * n1 * n2 * n3, st, in, n0, n1, n2, n3, out);
+}
+mk_status k_mpo_extend(const double* O, int wl, int d, int wr, int ne, int nq, double* out, hipStream_t st) {
+  LAUNCH_1D(mpo_extend_kernel, (int64_t)wl * ne * d * nq * d * nq * wr * ne, st, O, wl, d, wr, ne, nq, out);
 }
 mk_status k_tl_mid_mixed(const double* t1, const double* O, int64_t Dm, int wl, int d, int wr, int64_t Dc,
                          int64_t ldo, int split, void* hi, void* lo, hipStream_t st) {

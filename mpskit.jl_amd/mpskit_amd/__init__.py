@@ -39,5 +39,5 @@ __all__ += ["DMRG", "DMRG2", "AdaptiveKrylov", "Lanczos", "expectation_value", "
             "environments", "FiniteMPOHamiltonian", "FiniteMPS"]
 
 # algorithm modules beyond DMRG2 / VUMPS, reachable as mpskit_amd.<module>
-from . import approximate, blocks, boundary, changebonds, ed, excited, idmrg, observables, tdvp, vumps  # noqa: E402,F401
+from . import approximate, blocks, boundary, changebonds, ed, excited, idmrg, observables, qp, tdvp, vumps  # noqa: E402,F401
 

@@ -109,6 +109,11 @@ _SIGS = {
     "mk_prepare_transfer_right": ([_P, _P, _P, _P] + [_I64] * 7 + [_INT, C.POINTER(_P)], _INT),
     "mk_transfer_left_mpo": ([_P, _P, _P, _P, _P] + [_I64] * 7 + [_P], _INT),
     "mk_transfer_right_mpo": ([_P, _P, _P, _P, _P] + [_I64] * 7 + [_P], _INT),
+    "mk_transfer_left_excited": ([_P, _P, _P, _P, _P] + [_I64] * 8 + [_P], _INT),
+    "mk_transfer_right_excited": ([_P, _P, _P, _P, _P] + [_I64] * 8 + [_P], _INT),
+    "mk_transfer_left_leg": ([_P, _P, _P, _P, _P] + [_I64] * 8 + [_P], _INT),
+    "mk_transfer_right_leg": ([_P, _P, _P, _P, _P] + [_I64] * 8 + [_P], _INT),
+    "mk_mpo_extend": ([_P, _P] + [_I64] * 5 + [_P], _INT),
     "mk_regularize": ([_P, _P, _P, _P, _I64, _I64, _I64], _INT),
     "mk_eigsolve": ([_P, _P, _INT, _INT, _D, _INT, C.POINTER(_D), C.POINTER(_D), C.POINTER(_INT),
                      C.POINTER(_INT)], _INT),

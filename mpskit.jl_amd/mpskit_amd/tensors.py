@@ -93,6 +93,15 @@ def fuse(t: torch.Tensor, rows: int) -> torch.Tensor:
     return t.permute(*reversed(range(t.dim()))).reshape(n, m).t()
 
 
+def reshape(t: torch.Tensor, shape: Sequence[int]) -> torch.Tensor:
+    """Column-major reshape (legs fused or split with the first index fastest);
+    a view of a column-major tensor, else of a column-major copy."""
+    t = as_colmajor(t)
+    shape = tuple(int(n) for n in shape)
+    r = t.permute(*reversed(range(t.dim()))).reshape(tuple(reversed(shape)))
+    return r.permute(*reversed(range(len(shape)))) if len(shape) > 1 else r
+
+
 def unfuse(mat: torch.Tensor, shape: Sequence[int]) -> torch.Tensor:
     """Inverse of :func:`fuse` for a column-major matrix view."""
     shape = tuple(int(n) for n in shape)

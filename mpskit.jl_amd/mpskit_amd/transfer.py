@@ -4,8 +4,17 @@
 ``transfer_left(x, O, A, Abar)`` / ``transfer_right(v, O, A, Abar)`` for MPO
 environments, ``TransferMatrix(above, middle, below)`` with ``tm(v)``,
 ``v * tm`` (flipped, i.e. a left action), products and ``regularize``.
+
+The two functions take every argument pattern of the reference's method
+table (transfer.jl:18-161), told apart by tensor rank as there: an excited
+(quasiparticle) upper tensor A is (Dl, d, De, Dr), the reference's
+A[l p; e r]; legs of v between its first and last pass through; equal-rank
+A / Abar with several physical legs (density matrices, MPO pairs) contract
+them all.  Braidings are trivial (bosonic spaces).
 """
 from __future__ import annotations
+
+import math
 
 from . import tensors as T
 from ._lib import SpaceMismatch, check, load
@@ -17,28 +26,85 @@ def _shape_check(cond, msg):
         raise SpaceMismatch(1, msg)
 
 
+def _fuse_phys(A):
+    """(Dl, p1, ..., pk, Dr) -> (Dl, p1 ... pk, Dr), column-major."""
+    return T.reshape(A, (A.shape[0], math.prod(A.shape[1:-1]), A.shape[-1])) if A.dim() != 3 else A
+
+
 def transfer_left(v, *args):
-    """transfer_left(v, A, Abar) (transfer.jl:18-31) or transfer_left(x, O, A, Abar)
-    (transfer.jl:125-127); ``O is None`` falls back to the MPS-only form."""
+    """transfer_left(v, [O,] A, Abar) with the reference's dispatch:
+
+    - v (Dm, Dn), A, Abar MPS tensors: transfer.jl:18-31; several physical
+      legs on both (density matrices, MPO pairs :111-113) are contracted;
+    - v (Dm, g..., Dn): the legs g pass through (transfer.jl:18-31 with
+      N1 > 1, :76-79, :85-91, :101-104);
+    - A excited (Dn, d, De, Dc), Abar (Dm, d, Da): transfer.jl:58-60, :66-69;
+    - with an MPO O (wl, d, d, wr): x (Dm, wl, Dn) (:125-127), A excited
+      (:133-135), x (Dm, wl, De, Dn) with an excitation leg (:141-143), A and
+      Abar density-matrix tensors (Dn, d, dq, Dc) with O on the first
+      physical leg (:150-155); ``O is None`` is the MPS-only form (:122)."""
     if len(args) == 3:
         O, A, Ab = args
         if O is None:
             return transfer_left(v, A, Ab)
+        if A.dim() == 4 and Ab.dim() == 3 and v.dim() == 3:
+            return _tl_excited(v, O, A, Ab)
+        if A.dim() == 3 and Ab.dim() == 3 and v.dim() == 4:
+            return _tl_leg(v, O, A, Ab)
+        if A.dim() == 4 and Ab.dim() == 4 and v.dim() == 3:
+            return _tl_mpo(v, _mpo_extend(O, 1, A.shape[2]), _fuse_phys(A), _fuse_phys(Ab))
+        _shape_check(v.dim() == 3 and A.dim() == 3 and Ab.dim() == 3, "transfer_left(mpo): unsupported ranks")
         return _tl_mpo(v, O, A, Ab)
     A, Ab = args
-    return _tl_mps(v, A, Ab)
+    if A.dim() == Ab.dim() + 1 == 4:
+        _shape_check(v.dim() in (2, 3), "transfer_left(excited): v must be (Dm, Dn) or (Dm, g, Dn)")
+        if v.dim() == 2:
+            y = _tl_excited(T.reshape(v, (v.shape[0], 1, v.shape[1])), None, A, Ab)
+            return T.reshape(y, (y.shape[0], y.shape[2], y.shape[3]))
+        return _tl_excited(v, None, A, Ab)
+    _shape_check(A.dim() == Ab.dim() and A.dim() >= 3 and v.dim() >= 2, "transfer_left: unsupported ranks")
+    A, Ab = _fuse_phys(A), _fuse_phys(Ab)
+    if v.dim() == 2:
+        return _tl_mps(v, A, Ab)
+    g = v.shape[1:-1]
+    x = T.reshape(v, (v.shape[0], math.prod(g), 1, v.shape[-1]))
+    y = _tl_leg(x, None, A, Ab)
+    return T.reshape(y, (y.shape[0], *g, y.shape[-1]))
 
 
 def transfer_right(v, *args):
-    """transfer_right(v, A, Abar) (transfer.jl:43-55) or transfer_right(v, O, A, Abar)
-    (transfer.jl:128-130)."""
+    """transfer_right(v, [O,] A, Abar), mirror of :func:`transfer_left`:
+    transfer.jl:43-55 (passing legs of v: :80-83, :92-99, :105-108; MPO pairs
+    :114-116), excited A (Da, d, De, Di): :61-63, :70-73; with an MPO:
+    :128-130, excited A :136-138, v (Di, wr, De, Dj) :144-148, density
+    matrices :156-161."""
     if len(args) == 3:
         O, A, Ab = args
         if O is None:
             return transfer_right(v, A, Ab)
+        if A.dim() == 4 and Ab.dim() == 3 and v.dim() == 3:
+            return _tr_excited(v, O, A, Ab)
+        if A.dim() == 3 and Ab.dim() == 3 and v.dim() == 4:
+            return _tr_leg(v, O, A, Ab)
+        if A.dim() == 4 and Ab.dim() == 4 and v.dim() == 3:
+            return _tr_mpo(v, _mpo_extend(O, 1, A.shape[2]), _fuse_phys(A), _fuse_phys(Ab))
+        _shape_check(v.dim() == 3 and A.dim() == 3 and Ab.dim() == 3, "transfer_right(mpo): unsupported ranks")
         return _tr_mpo(v, O, A, Ab)
     A, Ab = args
-    return _tr_mps(v, A, Ab)
+    if A.dim() == Ab.dim() + 1 == 4:
+        _shape_check(v.dim() in (2, 3), "transfer_right(excited): v must be (Di, Dj) or (Di, g, Dj)")
+        if v.dim() == 2:
+            y = _tr_excited(T.reshape(v, (v.shape[0], 1, v.shape[1])), None, A, Ab)
+            return T.reshape(y, (y.shape[0], y.shape[2], y.shape[3]))
+        return _tr_excited(v, None, A, Ab)
+    _shape_check(A.dim() == Ab.dim() and A.dim() >= 3 and v.dim() >= 2, "transfer_right: unsupported ranks")
+    A, Ab = _fuse_phys(A), _fuse_phys(Ab)
+    if v.dim() == 2:
+        return _tr_mps(v, A, Ab)
+    g = v.shape[1:-1]
+    x = T.reshape(v, (v.shape[0], math.prod(g), 1, v.shape[-1]))
+    y = _tr_leg(x, None, A, Ab)
+    return T.reshape(y, (y.shape[0], *g, y.shape[-1]))
 
 
 def prepare_transfer_left(A, Ab, O=None, precisions=None):
@@ -139,6 +205,93 @@ def _tr_mpo(v, O, A, Ab):
     check(load().mk_transfer_right_mpo(ctx.handle, T.ptr(v), T.ptr(O), T.ptr(A), T.ptr(Ab), Da, d, Di, wl, wr, Dc,
                                        Do, T.ptr(y)))
     return y
+
+
+def _mpo_dims(O, x_leg, what):
+    """(wl, d, wr) of O, or of the identity on a passenger leg of size x_leg."""
+    if O is None:
+        return x_leg, None, x_leg
+    wl, d, d2, wr = O.shape
+    _shape_check(d == d2, what + ": MPO physical spaces differ")
+    return wl, d, wr
+
+
+def _tl_excited(x, O, A, Ab):
+    """y[a,b,e,c] = x[m,g,n] A[n,h,e,c] O[g,k,h,b] conj(Ab[m,k,a])
+    (``mk_transfer_left_excited``)."""
+    Dm, wl, Dn = x.shape
+    Dn2, d, De, Dc = A.shape
+    Dm2, d2, Da = Ab.shape
+    wl2, dO, wr = _mpo_dims(O, wl, "transfer_left(excited)")
+    _shape_check(Dn == Dn2 and Dm == Dm2 and d == d2 and wl == wl2 and dO in (None, d),
+                 "transfer_left(excited): incompatible spaces")
+    x, A, Ab = T.as_colmajor(x), T.as_colmajor(A), T.as_colmajor(Ab)
+    O = None if O is None else T.as_colmajor(O)
+    y = T.empty((Da, wr, De, Dc), x.device)
+    check(load().mk_transfer_left_excited(default_context().handle, T.ptr(x), None if O is None else T.ptr(O),
+                                          T.ptr(A), T.ptr(Ab), Dm, wl, Dn, d, De, wr, Dc, Da, T.ptr(y)))
+    return y
+
+
+def _tr_excited(v, O, A, Ab):
+    """y[a,b,e,c] = A[a,h,e,i] O[b,k,h,l] conj(Ab[c,k,o]) v[i,l,o]
+    (``mk_transfer_right_excited``)."""
+    Di, wr, Dj = v.shape
+    Da, d, De, Di2 = A.shape
+    Db, d2, Dj2 = Ab.shape
+    wl, dO, wr2 = _mpo_dims(O, wr, "transfer_right(excited)")
+    _shape_check(Di == Di2 and Dj == Dj2 and d == d2 and wr == wr2 and dO in (None, d),
+                 "transfer_right(excited): incompatible spaces")
+    v, A, Ab = T.as_colmajor(v), T.as_colmajor(A), T.as_colmajor(Ab)
+    O = None if O is None else T.as_colmajor(O)
+    y = T.empty((Da, wl, De, Db), v.device)
+    check(load().mk_transfer_right_excited(default_context().handle, T.ptr(v), None if O is None else T.ptr(O),
+                                           T.ptr(A), T.ptr(Ab), Da, d, De, Di, wl, wr, Db, Dj, T.ptr(y)))
+    return y
+
+
+def _tl_leg(x, O, A, Ab):
+    """y[a,b,e,c] = x[m,g,e,n] A[n,h,c] O[g,k,h,b] conj(Ab[m,k,a])
+    (``mk_transfer_left_leg``)."""
+    Dm, wl, De, Dn = x.shape
+    Dn2, d, Dc = A.shape
+    Dm2, d2, Da = Ab.shape
+    wl2, dO, wr = _mpo_dims(O, wl, "transfer_left(leg)")
+    _shape_check(Dn == Dn2 and Dm == Dm2 and d == d2 and wl == wl2 and dO in (None, d),
+                 "transfer_left(leg): incompatible spaces")
+    x, A, Ab = T.as_colmajor(x), T.as_colmajor(A), T.as_colmajor(Ab)
+    O = None if O is None else T.as_colmajor(O)
+    y = T.empty((Da, wr, De, Dc), x.device)
+    check(load().mk_transfer_left_leg(default_context().handle, T.ptr(x), None if O is None else T.ptr(O),
+                                      T.ptr(A), T.ptr(Ab), Dm, wl, De, Dn, d, wr, Dc, Da, T.ptr(y)))
+    return y
+
+
+def _tr_leg(v, O, A, Ab):
+    """y[a,b,e,c] = A[a,h,i] O[b,k,h,l] conj(Ab[c,k,o]) v[i,l,e,o]
+    (``mk_transfer_right_leg``)."""
+    Di, wr, De, Dj = v.shape
+    Da, d, Di2 = A.shape
+    Db, d2, Dj2 = Ab.shape
+    wl, dO, wr2 = _mpo_dims(O, wr, "transfer_right(leg)")
+    _shape_check(Di == Di2 and Dj == Dj2 and d == d2 and wr == wr2 and dO in (None, d),
+                 "transfer_right(leg): incompatible spaces")
+    v, A, Ab = T.as_colmajor(v), T.as_colmajor(A), T.as_colmajor(Ab)
+    O = None if O is None else T.as_colmajor(O)
+    y = T.empty((Da, wl, De, Db), v.device)
+    check(load().mk_transfer_right_leg(default_context().handle, T.ptr(v), None if O is None else T.ptr(O),
+                                       T.ptr(A), T.ptr(Ab), Da, d, Di, wl, wr, De, Db, Dj, T.ptr(y)))
+    return y
+
+
+def _mpo_extend(O, ne, nq):
+    """O widened by identities on a virtual (ne) and a physical (nq) leg
+    (``mk_mpo_extend``): out[(w,e),(k,q),(h,r),(v,f)] = O[w,k,h,v] d(e,f) d(q,r)."""
+    O = T.as_colmajor(O)
+    wl, d, _, wr = O.shape
+    out = T.empty((wl * ne, d * nq, d * nq, wr * ne), O.device)
+    check(load().mk_mpo_extend(default_context().handle, T.ptr(O), wl, d, wr, ne, nq, T.ptr(out)))
+    return out
 
 
 def regularize_(v, lvec, rvec):
