@@ -32,6 +32,7 @@ __all__ = [
     "tl_utility", "tr_utility", "tl_mpdo_utility", "tr_mpdo_utility", "tl_utility_excitation",
     "tr_utility_excitation", "tl_mpo_pair", "tr_mpo_pair", "tl_mpo_excited", "tr_mpo_excited",
     "tl_mpo_excitation_leg", "tr_mpo_excitation_leg", "tl_mpo_mpdo", "tr_mpo_mpdo", "finite_qp_environments",
+    "level_kinds", "infinite_qp_environments",
 ]
 
 
@@ -180,4 +181,140 @@ def finite_qp_environments(Bs, ALs, ARs, Ws, GL, GR):
     rB[L - 1] = np.zeros((ALs[-1].shape[2], Ws[-1].shape[3], De, ARs[-1].shape[2]))
     for p in range(L - 1, 0, -1):
         rB[p - 1] = tr_mpo_excitation_leg(rB[p], Ws[p], ALs[p], ARs[p]) + tr_mpo_excited(GR[p], Ws[p], Bs[p], ARs[p])
+    return lB, rB
+
+
+def level_kinds(Ws):
+    """Kind of each Jordan level of an infinite MPO cell (mpohamiltonian.jl:677-693):
+    "identity" for the first and last level and where every site holds the identity
+    on the diagonal, "empty" where some site has no diagonal block, else "diagonal"."""
+    w = Ws[0].shape[0]
+    kinds = []
+    for i in range(w):
+        blocks = [W[i, :, :, i] for W in Ws]
+        if i == 0 or i == w - 1 or all(np.array_equal(b, np.eye(b.shape[0])) for b in blocks):
+            kinds.append("identity")
+        elif any(not np.any(b) for b in blocks):
+            kinds.append("empty")
+        else:
+            kinds.append("diagonal")
+    return kinds
+
+
+def _reg(v, Cb):
+    """regularize!(v, C', C) on a level slice v[a, e, c] (transfermatrix.jl:79-84 with
+    lvec = C', rvec = C): v[a,e,c] -= (sum_ij C[j,i] v[j,e,i]) C[a,c]."""
+    lam = np.einsum("ji,jei->e", Cb, v)
+    return v - lam[None, :, None] * Cb[:, None, :]
+
+
+def _dense_solve(op, b, coef):
+    """x - coef * op(x) = b by a dense solve over the flattened space (the reference
+    runs GMRES, KrylovKit.linsolve(A, b, x0, alg, 1, -coef))."""
+    n = b.size
+    M = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        M[:, j] = e - coef * op(e.reshape(b.shape)).ravel()
+    return np.linalg.solve(M, b.ravel()).reshape(b.shape)
+
+
+def infinite_qp_environments(Bs, AL, AR, C, Ws, GL, GR, momentum):
+    """Environments of a quasiparticle on an infinite MPS with a Jordan MPO
+    Hamiltonian, trivial excitation (left_gs = right_gs), qp_envs.jl:49-137 with
+    left/right_excitation_transfer_system (exci_transfer_system.jl:1-100), for the
+    momenta whose phase cis(k) is real (k = 0, pi).  0-based sites: C[s] is the bond
+    right of site s (AL[s] C[s] = C[s-1] AR[s]); GL[s] / GR[s] the ground state's
+    environments of site s; lB[s] lives on the left bond of site s, rB[s] on the
+    right one.  l_RL(loc) = C[loc-1], r_RL(loc) = C[loc]', l_LR(loc) = C[loc-1]',
+    r_LR(loc) = C[loc] (infinitemps.jl:356-401) make every regularization here
+    regularize!(v, C', C) with the C of v's bond."""
+    assert abs(np.sin(momentum)) < 1e-12, "real phases only (momentum 0 or pi)"
+    p = float(np.cos(momentum))
+    N, w, De = len(Ws), Ws[0].shape[0], Bs[0].shape[2]
+    kinds = level_kinds(Ws)
+    ids = [i for i in range(1, w - 1) if kinds[i] == "identity"]
+    pN = p ** N
+    # qp_envs.jl:58-75
+    lB = [np.zeros((AL[s].shape[0], Ws[s].shape[0], De, AR[s].shape[0])) for s in range(N)]
+    for s in range(N):
+        y = (tl_mpo_excitation_leg(lB[s], Ws[s], AR[s], AL[s]) + tl_mpo_excited(GL[s], Ws[s], Bs[s], AL[s])) / p
+        for i in ids:
+            y[:, i] = _reg(y[:, i], C[s])
+        lB[(s + 1) % N] = y
+    # qp_envs.jl:77-90
+    rB = [np.zeros((AL[s].shape[2], Ws[s].shape[3], De, AR[s].shape[2])) for s in range(N)]
+    for s in range(N - 1, -1, -1):
+        y = (tr_mpo_excitation_leg(rB[s], Ws[s], AL[s], AR[s]) + tr_mpo_excited(GR[s], Ws[s], Bs[s], AR[s])) * p
+        for i in ids:
+            y[:, i] = _reg(y[:, i], C[s - 1])
+        rB[(s - 1) % N] = y
+    Cb = C[N - 1]
+
+    # left_excitation_transfer_system (exci_transfer_system.jl:1-48)
+    GBL, found = lB[0], np.zeros_like(lB[0])
+    for i in range(w):
+        x = found[:, : i + 1]
+        for s in range(N):
+            x = tl_mpo_excitation_leg(x, Ws[s][: i + 1, :, :, : i + 1], AR[s], AL[s])
+        start = x[:, i] * pN
+        if kinds[i] == "identity":
+            start = _reg(start, Cb)
+        found[:, i] = start + GBL[:, i]
+        if kinds[i] == "identity":
+            def op(v):
+                for s in range(N):
+                    v = tl_utility(v, AR[s], AL[s])
+                return _reg(v, Cb)
+        elif kinds[i] == "diagonal":
+            def op(v, i=i):
+                v = v[:, None]
+                for s in range(N):
+                    v = tl_mpo_excitation_leg(v, Ws[s][i:i + 1, :, :, i:i + 1], AR[s], AL[s])
+                return v[:, 0]
+        else:
+            continue
+        found[:, i] = _dense_solve(op, found[:, i], pN)
+    lB[0] = found
+
+    # right_excitation_transfer_system (exci_transfer_system.jl:50-100)
+    GBR, found = rB[N - 1], np.zeros_like(rB[N - 1])
+    for i in range(w - 1, -1, -1):
+        x = found[:, i:]
+        for s in range(N - 1, -1, -1):
+            x = tr_mpo_excitation_leg(x, Ws[s][i:, :, :, i:], AL[s], AR[s])
+        start = x[:, 0] * pN
+        if kinds[i] == "identity":
+            start = _reg(start, Cb)
+        found[:, i] = start + GBR[:, i]
+        if kinds[i] == "identity":
+            def op(v):
+                for s in range(N - 1, -1, -1):
+                    v = tr_utility(v, AL[s], AR[s])
+                return _reg(v, Cb)
+        elif kinds[i] == "diagonal":
+            def op(v, i=i):
+                v = v[:, None]
+                for s in range(N - 1, -1, -1):
+                    v = tr_mpo_excitation_leg(v, Ws[s][i:i + 1, :, :, i:i + 1], AL[s], AR[s])
+                return v[:, 0]
+        else:
+            continue
+        found[:, i] = _dense_solve(op, found[:, i], pN)
+    rB[N - 1] = found
+
+    # qp_envs.jl:110-136
+    cur = lB[0]
+    for s in range(N - 1):
+        cur = tl_mpo_excitation_leg(cur, Ws[s], AR[s], AL[s]) / p
+        for i in ids:
+            cur[:, i] = _reg(cur[:, i], C[s])
+        lB[s + 1] = lB[s + 1] + cur
+    cur = rB[N - 1]
+    for s in range(N - 1, 0, -1):
+        cur = tr_mpo_excitation_leg(cur, Ws[s], AL[s], AR[s]) * p
+        for i in ids:
+            cur[:, i] = _reg(cur[:, i], C[s - 1])
+        rB[s - 1] = rB[s - 1] + cur
     return lB, rB

@@ -205,3 +205,89 @@ def test_finite_qp_environments(mk, De):
         assert host(mk, lb[p]).shape == lB[p].shape and host(mk, rb[p]).shape == rB[p].shape
         assert rel(host(mk, lb[p]), lB[p]) < TOL if p > 0 else not np.any(host(mk, lb[p]))
         assert rel(host(mk, rb[p]), rB[p]) < TOL if p < L - 1 else not np.any(host(mk, rb[p]))
+
+
+def _uniform_host(N, d, D, rng):
+    """Uniform MPS on the host: AL left isometries, C (bond right of each site,
+    non-symmetric) with C C^T the AL transfer's right fixed point, AR = C^-1 AL C."""
+    AL = [np.linalg.qr(rng.standard_normal((D * d, D)))[0].reshape(D, d, D, order="F") for _ in range(N)]
+
+    def E(s, R):
+        return np.einsum("lhr,rs,mhs->lm", AL[s], R, AL[s])
+
+    R = np.eye(D) / D
+    for _ in range(5000):
+        Rn = R
+        for s in range(N - 1, -1, -1):
+            Rn = E(s, Rn)
+        Rn = Rn / np.trace(Rn)
+        done = np.linalg.norm(Rn - R) < 1e-15
+        R = Rn
+        if done:
+            break
+    Rs = [None] * N
+    Rs[N - 1] = R
+    for s in range(N - 1, 0, -1):
+        Rs[s - 1] = E(s, Rs[s])
+    C = []
+    for s in range(N):
+        lam, V = np.linalg.eigh(0.5 * (Rs[s] + Rs[s].T))
+        U = np.linalg.qr(rng.standard_normal((D, D)))[0]
+        C.append(((V * np.sqrt(lam)) @ V.T) @ U)
+    AR = [np.einsum("lm,mpr,rk->lpk", np.linalg.inv(C[s - 1]), AL[s], C[s]) for s in range(N)]
+    return AL, AR, C
+
+
+@pytest.mark.parametrize("model", ["xxz", "exp_ising", "exp_ising_uniform"])
+@pytest.mark.parametrize("N,De", [(1, 2), (2, 1)])
+@pytest.mark.parametrize("momentum", [0.0, np.pi])
+def test_infinite_qp_environments(mk, model, N, De, momentum):
+    """``qp.infinite_qp_environments`` (qp_envs.jl:49-137; GMRES on the HIP transfers)
+    against the oracle's dense per-level solves, 1e-9 relative: both momenta with a
+    real phase, cells of 1 and 2 sites, excitation legs of 1 and 2, XXZ (empty
+    interior levels), exponential Ising (a diagonal level) and its lambda = 1 form
+    (an interior identity level)."""
+    from types import SimpleNamespace
+    from mpskit_amd import models as M
+    from mpskit_amd import vumps as V
+    print("infinite qp environments", model, N, De, momentum, flush=True)
+    rng = np.random.default_rng(7 * N + De + len(model))
+    d, D = 2, 12
+    W = {"xxz": lambda: M.xxz_bulk(0.7), "exp_ising": lambda: M.exp_ising_bulk(1.0, 0.5, 0.8),
+         "exp_ising_uniform": lambda: M.exp_ising_bulk(1.0, 1.0, 0.8)}[model]()
+    Ws = [W] * N
+    w = W.shape[0]
+    AL, AR, C = _uniform_host(N, d, D, rng)
+    GL = [rng.standard_normal((D, w, D)) for _ in range(N)]
+    GR = [rng.standard_normal((D, w, D)) for _ in range(N)]
+    Bs = [rng.standard_normal((D, d, De, D)) for _ in range(N)]
+    lB, rB = Q.infinite_qp_environments(Bs, AL, AR, C, Ws, GL, GR, momentum)
+    up = lambda xs: [dev(mk, x) for x in xs]  # noqa: E731
+    psi = V.InfiniteMPS(up(AL), up(AR), up(C))
+    envs = SimpleNamespace(GLs=up(GL), GRs=up(GR))
+    lb, rb = mk.qp.infinite_qp_environments(up(Bs), psi, V.InfiniteHamiltonian(Ws), envs, momentum)
+    for s in range(N):
+        assert rel(host(mk, lb[s]), lB[s]) < 1e-9, s
+        assert rel(host(mk, rb[s]), rB[s]) < 1e-9, s
+
+
+def test_infinite_qp_environments_of_a_ground_state_environment(mk):
+    """The same on a random uniform MPS with its own infinite environments
+    (``InfiniteEnvironments``: per-level GMRES fixed points), TFIM, D = 16, cell 2."""
+    from mpskit_amd import models as M
+    from mpskit_amd import vumps as V
+    print("infinite qp environments, ground-state environments", flush=True)
+    rng = np.random.default_rng(11)
+    Ws = [M.tfim_bulk(0.8)] * 2
+    H = V.InfiniteHamiltonian(Ws)
+    psi = V.InfiniteMPS.random(2, 2, 16, seed=4)
+    envs = V.InfiniteEnvironments(psi, H)
+    Bs = [rng.standard_normal((A.shape[0], 2, 1, A.shape[2])) for A in psi.AL]
+    lb, rb = mk.qp.infinite_qp_environments([dev(mk, b) for b in Bs], psi, H, envs, 0.0)
+    hostl = lambda xs: [host(mk, x) for x in xs]  # noqa: E731
+    lB, rB = Q.infinite_qp_environments(Bs, hostl(psi.AL), hostl(psi.AR), hostl(psi.C), Ws, hostl(envs.GLs),
+                                        hostl(envs.GRs), 0.0)
+    for s in range(2):
+        assert rel(host(mk, lb[s]), lB[s]) < 1e-9 and rel(host(mk, rb[s]), rB[s]) < 1e-9
+    with pytest.raises(NotImplementedError):
+        mk.qp.infinite_qp_environments([dev(mk, b) for b in Bs], psi, H, envs, 0.3)

@@ -8,6 +8,7 @@ physical leg under O (x) 1 built by loops; left and right variants are tied by
 the ring identity; a planted leg exchange fails the pins.
 """
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 from oracle import transfer_qp as Q
@@ -232,3 +233,147 @@ def test_finite_qp_environments_vs_dense_blocks():
         R = sum(_ket_right(AL, AR, Bs, j, p) for j in range(p + 1, L))
         want = np.einsum("ieS,wTS,bT->iweb", R, _mpo_right(Ws[p + 1:]), V)
         assert rel(rB[p], want) < 1e-12
+
+
+# -- infinite quasiparticle environments -----------------------------------------
+
+def _uniform_mps(N, d, D, rng):
+    """AL (left isometries), C (bond right of each site, C C^T the right fixed point of
+    the AL transfer, |C| = 1, not symmetric) and AR = C[s-1]^-1 AL C[s], so
+    AL[s] C[s] = C[s-1] AR[s] and AR is right-isometric."""
+    AL = []
+    for _ in range(N):
+        Q, _ = np.linalg.qr(rng.standard_normal((D * d, D)))
+        AL.append(Q.reshape(D, d, D, order="F"))
+
+    def E(s, R):                                   # right action of site s: sum_h AL_h R AL_h^T
+        return np.einsum("lhr,rs,mhs->lm", AL[s], R, AL[s])
+
+    R = np.eye(D) / D
+    for _ in range(5000):
+        Rn = R
+        for s in range(N - 1, -1, -1):
+            Rn = E(s, Rn)
+        Rn = Rn / np.trace(Rn)
+        if np.linalg.norm(Rn - R) < 1e-15:
+            break
+        R = Rn
+    Rs = [None] * N
+    Rs[N - 1] = R
+    for s in range(N - 1, 0, -1):
+        Rs[s - 1] = E(s, Rs[s])
+    C = []
+    for Rb in Rs:
+        lam, V = np.linalg.eigh(0.5 * (Rb + Rb.T))
+        C.append((V * np.sqrt(lam)) @ V.T)
+    # a random orthogonal gauge on every bond makes C non-symmetric (C -> C U, AR -> U^T AR U)
+    U = [np.linalg.qr(rng.standard_normal((D, D)))[0] for _ in range(N)]
+    C = [C[s] @ U[s] for s in range(N)]
+    AR = [np.einsum("lm,mpr,rk->lpk", np.linalg.inv(C[s - 1]), AL[s], C[s]) for s in range(N)]
+    return AL, AR, C
+
+
+def _series_left(S, AL, AR, C, Ws, pN, kinds):
+    """x = Reg_id(p^N x T_cell) + S by fixed-point iteration of the whole cell map."""
+    N, x = len(Ws), np.zeros_like(S)
+    for it in range(20000):
+        y = x
+        for s in range(N):
+            y = Q.tl_mpo_excitation_leg(y, Ws[s], AR[s], AL[s])
+        y = pN * y
+        for i, k in enumerate(kinds):
+            if k == "identity":
+                y[:, i] = Q._reg(y[:, i], C[N - 1])
+        xn = y + S
+        if np.linalg.norm(xn - x) <= 1e-15 * max(np.linalg.norm(xn), 1.0):
+            return xn, it
+        x = xn
+    raise AssertionError("series did not converge")
+
+
+def _series_right(S, AL, AR, C, Ws, pN, kinds):
+    N, x = len(Ws), np.zeros_like(S)
+    for it in range(20000):
+        y = x
+        for s in range(N - 1, -1, -1):
+            y = Q.tr_mpo_excitation_leg(y, Ws[s], AL[s], AR[s])
+        y = pN * y
+        for i, k in enumerate(kinds):
+            if k == "identity":
+                y[:, i] = Q._reg(y[:, i], C[N - 1])
+        xn = y + S
+        if np.linalg.norm(xn - x) <= 1e-15 * max(np.linalg.norm(xn), 1.0):
+            return xn, it
+        x = xn
+    raise AssertionError("series did not converge")
+
+
+@pytest.mark.parametrize("model", ["xxz", "exp_ising", "exp_ising_uniform"])
+@pytest.mark.parametrize("N", [1, 2])
+@pytest.mark.parametrize("momentum", [0.0, np.pi])
+def test_infinite_qp_environments_solve_the_cell_recursion(model, N, momentum):
+    """The per-level triangular solves (exci_transfer_system.jl) against the fixed point
+    of the whole-cell map x = Reg(p^N x T_cell) + S iterated to convergence, where S is
+    the cell recursion of the excited tensors from zero (qp_envs.jl:58-90); the
+    regularizing C are checked to be the fixed points of the mixed transfers.  Models:
+    XXZ (empty interior levels), exponential Ising (a diagonal level) and its lambda = 1
+    version (an interior identity level, regularized inside the recursion)."""
+    from mpskit_amd import models as M
+    rng = np.random.default_rng(N * 10 + int(momentum) + len(model))
+    d, D, De = 2, 4, 2
+    W = {"xxz": lambda: M.xxz_bulk(0.7), "exp_ising": lambda: M.exp_ising_bulk(1.0, 0.5, 0.8),
+         "exp_ising_uniform": lambda: M.exp_ising_bulk(1.0, 1.0, 0.8)}[model]()
+    Ws = [W] * N
+    w = W.shape[0]
+    kinds = Q.level_kinds(Ws)
+    if model == "exp_ising_uniform":
+        assert "identity" in kinds[1:-1]
+    AL, AR, C = _uniform_mps(N, d, D, rng)
+    # the regularizations' vectors: C is the left fixed point of x T(AR, AL) and the right one of T(AL, AR)
+    x, y = C[N - 1], C[N - 1]
+    for s in range(N):
+        x = O.transfer_left_mps(x, AR[s], AL[s])
+    for s in range(N - 1, -1, -1):
+        y = O.transfer_right_mps(y, AL[s], AR[s])
+    assert rel(x, C[N - 1]) < 1e-12 and rel(y, C[N - 1]) < 1e-12
+    # and C^T the other fixed point of each: the right one of T(AR, AL), the left one of T(AL, AR)
+    x, y = C[N - 1].T, C[N - 1].T
+    for s in range(N - 1, -1, -1):
+        x = O.transfer_right_mps(x, AR[s], AL[s])
+    for s in range(N):
+        y = O.transfer_left_mps(y, AL[s], AR[s])
+    assert rel(x, C[N - 1].T) < 1e-12 and rel(y, C[N - 1].T) < 1e-12
+    assert abs(np.trace(C[N - 1].T @ C[N - 1]) - 1.0) < 1e-12
+    # regularize! is the projection that removes the fixed point C along its dual C^T:
+    # Reg(C) = 0 and tr(Reg(v)_e C^T ... ) pairs to zero with the dual
+    v = rng.standard_normal((D, De, D))
+    Cb = C[N - 1]
+    assert np.abs(Q._reg(np.repeat(Cb[:, None, :], De, axis=1), Cb)).max() < 1e-13
+    assert np.abs(np.einsum("aec,ca->e", Q._reg(v, Cb), Cb.T)).max() < 1e-13
+    GL = [rng.standard_normal((D, w, D)) for _ in range(N)]
+    GR = [rng.standard_normal((D, w, D)) for _ in range(N)]
+    Bs = [rng.standard_normal((D, d, De, D)) for _ in range(N)]
+    lB, rB = Q.infinite_qp_environments(Bs, AL, AR, C, Ws, GL, GR, momentum)
+    p = float(np.cos(momentum))
+    ids = [i for i in range(1, w - 1) if kinds[i] == "identity"]
+    # S: the cell recursion of the excited tensors from zero
+    Sl = np.zeros((D, w, De, D))
+    for s in range(N):
+        Sl = (Q.tl_mpo_excitation_leg(Sl, Ws[s], AR[s], AL[s]) + Q.tl_mpo_excited(GL[s], Ws[s], Bs[s], AL[s])) / p
+        for i in ids:
+            Sl[:, i] = Q._reg(Sl[:, i], C[s])
+    Sr = np.zeros((D, w, De, D))
+    for s in range(N - 1, -1, -1):
+        Sr = (Q.tr_mpo_excitation_leg(Sr, Ws[s], AL[s], AR[s]) + Q.tr_mpo_excited(GR[s], Ws[s], Bs[s], AR[s])) * p
+        for i in ids:
+            Sr[:, i] = Q._reg(Sr[:, i], C[s - 1])
+    xl, _ = _series_left(Sl, AL, AR, C, Ws, p ** N, kinds)
+    xr, _ = _series_right(Sr, AL, AR, C, Ws, p ** N, kinds)
+    assert rel(lB[0], xl) < 1e-10
+    assert rel(rB[N - 1], xr) < 1e-10
+    # the other sites carry the same environments moved through the cell
+    for s in range(N - 1):
+        y = (Q.tl_mpo_excitation_leg(lB[s], Ws[s], AR[s], AL[s]) + Q.tl_mpo_excited(GL[s], Ws[s], Bs[s], AL[s])) / p
+        for i in ids:
+            y[:, i] = Q._reg(y[:, i], C[s])
+        assert rel(lB[s + 1], y) < 1e-10
