@@ -157,6 +157,48 @@ def apply_table(mk, T, steps=5):
     return out
 
 
+def qp_transfer_table(mk, T, D=1024, De=2, reps=10):
+    """Times of the quasiparticle and excitation-leg MPO transfers (transfer.jl:133-148)
+    beside the plain MPO transfer (transfer.jl:125-130), D = 1024, w = 5, d = 2, an
+    excitation leg of De = 2, FP64.  Each variant is the plain transfer's two GEMMs with
+    one bond or MPO leg De times wider, so its algorithmic flops are De times the plain
+    transfer's 2 D^3 d (wl + wr); the rate shows what the reshapes, the leg swap of the
+    excited right transfer and the widened MPO step cost on top."""
+    import torch
+    rng = np.random.default_rng(21)
+    w, d = 5, 2
+    W = T.from_numpy(jordan_w(rng))
+    x = gpu_unit(T, (D, w, D), 31)
+    x4 = gpu_unit(T, (D, w, De, D), 32)
+    A, Ab = gpu_unit(T, (D, d, D), 33), gpu_unit(T, (D, d, D), 34)
+    B = gpu_unit(T, (D, d, De, D), 35)
+    base = 2.0 * D ** 3 * d * (2 * w)
+    cases = {
+        "transfer_left(x, O, A, Ab)": (lambda: mk.transfer_left(x, W, A, Ab), base),
+        "transfer_right(v, O, A, Ab)": (lambda: mk.transfer_right(x, W, A, Ab), base),
+        "transfer_left(x, O, B, Ab), excited A": (lambda: mk.transfer_left(x, W, B, Ab), De * base),
+        "transfer_right(v, O, B, Ab), excited A": (lambda: mk.transfer_right(x, W, B, Ab), De * base),
+        "transfer_left(x4, O, A, Ab), excitation leg": (lambda: mk.transfer_left(x4, W, A, Ab), De * base),
+        "transfer_right(v4, O, A, Ab), excitation leg": (lambda: mk.transfer_right(x4, W, A, Ab), De * base),
+    }
+    out = {"config": f"D={D}, w={w}, d={d}, De={De}, float64"}
+    for name, (fn, flops) in cases.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = dict(ms=ms, tflops=flops / (ms * 1e-3) / 1e12)
+        print(f"# {name}: {ms:.3f} ms, {out[name]['tflops']:.1f} TFLOP/s", file=sys.stderr, flush=True)
+    del x, x4, A, Ab, B
+    torch.cuda.empty_cache()
+    return out
+
+
 def batched_apply(mk, T, reps=10):
     """The grouped (batched) dAC2 path: one mk_batch_apply applies every
     instance of a batch in one grouped persistent GEMM launch plus one memory-
@@ -820,6 +862,8 @@ def main():
     if not args.no_table:
         print("# apply table ...", file=sys.stderr, flush=True)
         out["apply_table"] = apply_table(mk, T)
+        print("# quasiparticle transfers ...", file=sys.stderr, flush=True)
+        out["qp_transfers"] = qp_transfer_table(mk, T)
     if not args.no_sweep:
         sw = {}
         for prec in ("fp64", "bf16x3"):
